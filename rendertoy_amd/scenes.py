@@ -1,0 +1,420 @@
+"""Deterministic scene generators for the workloads this package is measured on.
+
+A scene is a flat record of what ``RenderToy::World`` holds once assembled
+(include/RenderToy/world.h:13-41): world-space triangle vertices in
+``World::triangles`` order, per-vertex normals, the owning mesh of each
+triangle, per-mesh material and smooth flag, the material table and the
+emitter list in ``World::PrepareDirectLightSampling`` order
+(src/world.cpp:24-37).  Cameras carry the object-to-world matrix and the
+``Camera`` lens fields (include/RenderToy/object.h:69-95).
+
+Generators use numpy's PCG64 stream, float64 geometry and one final cast to
+float32; a sha256 digest of the vertex buffer is recorded so any drift is visible.
+
+Workloads (sizes fixed by the benchmark definitions):
+  * ``cornell_c0``   -- 36-triangle Cornell box, 256x256, 16 spp, depth 5.
+  * ``spheres_c1``   -- 64 spheres x 16,384 triangles = 1,048,576 triangles,
+                        1920x1080 primary rays.  16,384 = 4*4^6, a geodesic
+                        tetrahedron subdivided six times (an icosahedron cannot
+                        give 16,384).
+  * ``terrain_c2``   -- 1024x1024-cell Perlin heightfield (2,097,152 triangles)
+                        plus 256 icospheres of 1,280 triangles (327,680), 32
+                        Disney materials, 3 area quads; 1920x1080, 64 spp, depth 8.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MAT_FLOATS = 24
+NONE = 0xFFFFFFFF
+
+# Field order of one material record (PrincipledBSDF, include/RenderToy/material.h:12-77).
+MAT_FIELDS = ("base_color", "emission", "roughness", "metallic", "spec_trans", "ior",
+              "anisotropic", "subsurface", "specular_tint", "sheen", "sheen_tint",
+              "clearcoat", "clearcoat_roughness", "at_distance", "extinction")
+
+
+def principled_bsdf(base_color, emission=(0.0, 0.0, 0.0), roughness=0.5, metallic=0.0,
+                    spec_trans=0.0, ior=1.45, anisotropic=0.0, subsurface=0.0,
+                    specular_tint=0.0, sheen=0.0, sheen_tint=0.0, clearcoat=0.0,
+                    clearcoat_roughness=0.0, at_distance=1.0, extinction=(1.0, 1.0, 1.0)):
+    """One material record with PrincipledBSDF's constructor defaults and its
+    roughness clamp at 0.001 (src/material.cpp:12-33)."""
+    m = np.zeros(MAT_FLOATS, np.float32)
+    m[0:3] = base_color
+    m[3:6] = emission
+    r = np.float32(roughness)
+    m[6] = np.float32(0.001) if r < np.float32(0.001) else r
+    m[7:18] = (metallic, spec_trans, ior, anisotropic, subsurface, specular_tint, sheen,
+               sheen_tint, clearcoat, clearcoat_roughness, at_distance)
+    m[18:21] = extinction
+    return m
+
+
+@dataclass
+class Camera:
+    o2w: np.ndarray                       # (16,) float32, row-major
+    focal: float = 35.0                   # mm
+    gate: tuple = (0.868, 0.631)          # inches, academy_format (object.h:91)
+    near: float = 0.1
+    far: float = 1000.0
+
+    def screen_space(self, W: int, H: int):
+        """Renderer::PrepareScreenSpace (src/renderer.cpp:103-124) in float32."""
+        f32 = np.float32
+        gx, gy, fl = f32(self.gate[0]), f32(self.gate[1]), f32(self.focal)
+        top = (gy * f32(25.4) / f32(2.0)) / fl
+        right = (gx * f32(25.4) / f32(2.0)) / fl
+        gate_aspr = gx / gy
+        film_aspr = f32(W) / f32(H)
+        if gate_aspr >= film_aspr:
+            right = right * (film_aspr / gate_aspr)
+        else:
+            top = top * (gate_aspr / film_aspr)
+        return f32(top), f32(right)
+
+
+@dataclass
+class Scene:
+    name: str
+    verts: np.ndarray            # (n,3,3) float32
+    norms: np.ndarray            # (n,3,3) float32
+    tri_mesh: np.ndarray         # (n,) uint32
+    mesh_mat: np.ndarray         # (m,) uint32
+    mesh_smooth: np.ndarray      # (m,) uint8
+    mats: np.ndarray             # (k,24) float32
+    emitters: np.ndarray         # (e,) uint32
+    camera: Camera
+    width: int
+    height: int
+    spp: int = 1
+    max_depth: int = 5
+    sky: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+    ground: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.verts.shape[0])
+
+    def fingerprint(self) -> str:
+        return digest(self.verts.tobytes())
+
+
+def digest(data: bytes) -> str:
+    """sha256 of a buffer, recorded with each generated scene."""
+    return hashlib.sha256(data).hexdigest()[:16]
+
+
+def emitter_list(mesh_of_tri: np.ndarray, mesh_mat: np.ndarray, mats: np.ndarray) -> np.ndarray:
+    """World::PrepareDirectLightSampling (src/world.cpp:24-37): every triangle of
+    every mesh whose emission differs from zero by more than 1e-7 (the Vector
+    operator== test, rtmath.h:296-308), meshes in order, triangles in
+    Mesh::tris order (append order here)."""
+    out = []
+    for m in range(len(mesh_mat)):
+        e = mats[mesh_mat[m], 3:6]
+        if np.any(np.abs(e - np.float32(0.0)) > np.float32(1e-7)):
+            out.append(np.nonzero(mesh_of_tri == m)[0])
+    if not out:
+        return np.zeros(0, np.uint32)
+    return np.concatenate(out).astype(np.uint32)
+
+
+def _finish(name, tris, norms, tri_mesh, mesh_mat, mesh_smooth, mats, camera, W, H, spp, depth):
+    v = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 3, 3))
+    v = v + np.float32(0.0)        # canonical +0 as Mesh::O2WTransform yields (object.cpp:42-48)
+    n = np.ascontiguousarray(np.asarray(norms, np.float32).reshape(-1, 3, 3))
+    tri_mesh = np.ascontiguousarray(tri_mesh, np.uint32)
+    mesh_mat = np.ascontiguousarray(mesh_mat, np.uint32)
+    mesh_smooth = np.ascontiguousarray(mesh_smooth, np.uint8)
+    mats = np.ascontiguousarray(np.asarray(mats, np.float32).reshape(-1, MAT_FLOATS))
+    em = emitter_list(tri_mesh, mesh_mat, mats)
+    return Scene(name, v, n, tri_mesh, mesh_mat, mesh_smooth, mats, em, camera, W, H, spp, depth)
+
+
+# ----------------------------------------------------------------------------- cameras
+
+def _rot_z(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[c, -s, 0, 0], [s, c, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], np.float64)
+
+
+def _rot_x(a):
+    c, s = math.cos(a), math.sin(a)
+    return np.array([[1, 0, 0, 0], [0, c, -s, 0], [0, s, c, 0], [0, 0, 0, 1]], np.float64)
+
+
+def _trans(t):
+    m = np.eye(4)
+    m[:3, 3] = t
+    return m
+
+
+def look_camera(position, yaw_deg, pitch_deg, focal, gate=(0.868, 0.631)) -> Camera:
+    """Camera at `position`; with yaw 0 / pitch 0 it looks along +x with +z up
+    (T * Rz(yaw-90) * Rx(90+pitch), the demo's Euler convention)."""
+    m = _trans(position) @ _rot_z(math.radians(yaw_deg - 90.0)) @ _rot_x(math.radians(90.0 + pitch_deg))
+    return Camera(np.ascontiguousarray(m.astype(np.float32).reshape(16)), float(focal), gate)
+
+
+# ----------------------------------------------------------------------------- meshes
+
+def _quad(a, b, c, d):
+    """Fan-triangulated quad (Polygon::ConvertToTriangle, src/object.cpp:333-342)."""
+    return [(a, b, c), (a, c, d)]
+
+
+def _box_tris(cx, cy, sx, sy, sz, angle_deg):
+    ca, sa = math.cos(math.radians(angle_deg)), math.sin(math.radians(angle_deg))
+
+    def p(x, y, z):
+        return (cx + ca * x - sa * y, cy + sa * x + ca * y, z)
+    hx, hy = sx / 2, sy / 2
+    c = [p(-hx, -hy, 0), p(hx, -hy, 0), p(hx, hy, 0), p(-hx, hy, 0),
+         p(-hx, -hy, sz), p(hx, -hy, sz), p(hx, hy, sz), p(-hx, hy, sz)]
+    faces = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+    out = []
+    for f in faces:
+        out += _quad(c[f[0]], c[f[1]], c[f[2]], c[f[3]])
+    return out
+
+
+def cornell_c0(W=256, H=256, spp=16, max_depth=5) -> Scene:
+    """Cornell box assembled as demo/CornellBox does (box [-1,1]x[-1,1]x[0,2],
+    z up, camera looking +x through the open side), 36 triangles:
+    floor, ceiling, back, left (red), right (green) walls, a 0.5x0.5 light quad
+    with emission 15 just under the ceiling, a 0.6x0.6x1.2 block at +15 deg and a
+    0.6x0.6x0.6 block at -18 deg.  Pinhole camera, 40 deg vertical field of view."""
+    white = principled_bsdf((0.73, 0.73, 0.73))
+    red = principled_bsdf((0.65, 0.05, 0.05))
+    green = principled_bsdf((0.12, 0.45, 0.15))
+    light = principled_bsdf((0.73, 0.73, 0.73), emission=(15.0, 15.0, 15.0))
+    mats = [white, red, green, light]
+    meshes = []   # (tris, material)
+    meshes.append((_quad((-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0)), 0))      # floor
+    meshes.append((_quad((-1, -1, 2), (-1, 1, 2), (1, 1, 2), (1, -1, 2)), 0))      # ceiling
+    meshes.append((_quad((1, -1, 0), (1, -1, 2), (1, 1, 2), (1, 1, 0)), 0))        # back wall
+    meshes.append((_quad((-1, 1, 0), (1, 1, 0), (1, 1, 2), (-1, 1, 2)), 1))        # left wall (+y)
+    meshes.append((_quad((-1, -1, 0), (-1, -1, 2), (1, -1, 2), (1, -1, 0)), 2))    # right wall (-y)
+    lz = 1.98
+    meshes.append((_quad((-0.25, -0.25, lz), (-0.25, 0.25, lz), (0.25, 0.25, lz), (0.25, -0.25, lz)), 3))
+    meshes.append((_box_tris(0.30, 0.35, 0.6, 0.6, 1.2, 15.0), 0))               # tall block
+    meshes.append((_box_tris(-0.35, -0.30, 0.6, 0.6, 0.6, -18.0), 0))            # short block
+    tris, tri_mesh, mesh_mat = [], [], []
+    for mi, (tl, mat) in enumerate(meshes):
+        tris += tl
+        tri_mesh += [mi] * len(tl)
+        mesh_mat.append(mat)
+    tris = np.array(tris, np.float64)
+    # flat meshes: per-vertex normals are unused but carried as the importer does
+    norms = np.zeros_like(tris)
+    focal = (0.631 * 25.4 / 2.0) / math.tan(math.radians(20.0))
+    cam = look_camera((-3.9, 0.0, 1.0), 0.0, 0.0, focal)
+    return _finish("cornell_c0", tris, norms, tri_mesh, mesh_mat, [0] * len(meshes), mats, cam,
+                   W, H, spp, max_depth)
+
+
+def geodesic_tetra(level: int):
+    """Unit geodesic sphere from a tetrahedron: 4*4^level triangles, indexed."""
+    v = [np.array(p, np.float64) / math.sqrt(3.0) for p in
+         [(1, 1, 1), (-1, -1, 1), (-1, 1, -1), (1, -1, -1)]]
+    f = [(0, 1, 2), (0, 3, 1), (0, 2, 3), (1, 3, 2)]
+    return _subdivide(v, f, level)
+
+
+def icosphere(level: int):
+    """Unit icosphere: 20*4^level triangles, indexed."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    raw = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+           (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    v = [np.array(p, np.float64) / math.sqrt(1 + t * t) for p in raw]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    return _subdivide(v, f, level)
+
+
+def _subdivide(v, f, level):
+    verts = np.array(v, np.float64)
+    faces = np.array(f, np.int64)
+    for _ in range(level):
+        nv = len(verts)
+        e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+        key = np.sort(e, axis=1)
+        uniq, inv = np.unique(key[:, 0] * (1 << 32) + key[:, 1], return_inverse=True)
+        a, b = uniq >> 32, uniq & 0xFFFFFFFF
+        mid = verts[a] + verts[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        verts = np.concatenate([verts, mid])
+        nf = len(faces)
+        m01 = nv + inv[:nf]
+        m12 = nv + inv[nf:2 * nf]
+        m20 = nv + inv[2 * nf:]
+        f0, f1, f2 = faces[:, 0], faces[:, 1], faces[:, 2]
+        faces = np.concatenate([np.stack([f0, m01, m20], 1), np.stack([f1, m12, m01], 1),
+                                np.stack([f2, m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return verts, faces
+
+
+def _palette(rng, k, emissive=False):
+    mats = []
+    for _ in range(k):
+        mats.append(principled_bsdf(tuple(rng.uniform(0.1, 0.9, 3))))
+    return mats
+
+
+def spheres_c1(W=1920, H=1080, n_spheres=64, level=6, seed=1, smooth=True) -> Scene:
+    """64 jittered geodesic spheres, centres U[-10,10]^3, radii U[0.3,2.0],
+    per-vertex Gaussian jitter sigma = 0.01*radius; camera 30 units back."""
+    rng = np.random.default_rng(seed)
+    uv, uf = geodesic_tetra(level)
+    centers = rng.uniform(-10.0, 10.0, (n_spheres, 3))
+    radii = rng.uniform(0.3, 2.0, n_spheres)
+    mats = _palette(rng, 8)
+    tris, norms, tri_mesh, mesh_mat = [], [], [], []
+    for i in range(n_spheres):
+        p = uv * radii[i] + rng.normal(0.0, 0.01 * radii[i], uv.shape)
+        n = p / np.linalg.norm(p, axis=1, keepdims=True)
+        p = p + centers[i]
+        tris.append(p[uf])
+        norms.append(n[uf])
+        tri_mesh.append(np.full(len(uf), i, np.uint32))
+        mesh_mat.append(i % len(mats))
+    tris = np.concatenate(tris)
+    norms = np.concatenate(norms)
+    cam = look_camera((0.0, -30.0, 0.0), 90.0, 0.0, 26.0)
+    return _finish("spheres_c1", tris, norms, np.concatenate(tri_mesh), mesh_mat,
+                   [1 if smooth else 0] * n_spheres, mats, cam, W, H, 1, 1)
+
+
+# ----------------------------------------------------------------------------- Perlin terrain
+
+def _perlin2(x, y, perm):
+    """Improved Perlin noise (Perlin 2002) in 2D, vectorised; perm has 512 entries."""
+    xi = np.floor(x).astype(np.int64) & 255
+    yi = np.floor(y).astype(np.int64) & 255
+    xf = x - np.floor(x)
+    yf = y - np.floor(y)
+
+    def fade(t):
+        return t * t * t * (t * (t * 6 - 15) + 10)
+
+    def grad(h, dx, dy):
+        h = h & 7
+        u = np.where(h < 4, dx, dy)
+        v = np.where(h < 4, dy, dx)
+        return np.where(h & 1, -u, u) + np.where(h & 2, -2.0 * v, 2.0 * v)
+    u, v = fade(xf), fade(yf)
+    aa = perm[perm[xi] + yi]
+    ab = perm[perm[xi] + yi + 1]
+    ba = perm[perm[xi + 1] + yi]
+    bb = perm[perm[xi + 1] + yi + 1]
+    x1 = grad(aa, xf, yf) * (1 - u) + grad(ba, xf - 1, yf) * u
+    x2 = grad(ab, xf, yf - 1) * (1 - u) + grad(bb, xf - 1, yf - 1) * u
+    return (x1 * (1 - v) + x2 * v) * 0.5
+
+
+def terrain_c2(W=1920, H=1080, spp=64, max_depth=8, cells=1024, n_spheres=256, sphere_level=3,
+               n_mats=32, seed=2) -> Scene:
+    """Heightfield of cells x cells quads (2 triangles each) over [-50,50]^2,
+    6-octave Perlin noise of amplitude 2, plus n_spheres icospheres of
+    20*4^sphere_level triangles, 32 random Disney materials and 3 emissive quads.
+    Disney "specular 0.5" (F0 = 0.04) is expressed as ior 1.5; clearcoat uses
+    clearcoat_roughness 0.03 (PrincipledBSDF's member default)."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(256)
+    perm = np.concatenate([perm, perm]).astype(np.int64)
+    g = np.linspace(-50.0, 50.0, cells + 1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    Z = np.zeros_like(X)
+    amp, freq = 2.0, 1.0 / 25.0
+    for _ in range(6):
+        Z += amp * _perlin2(X * freq + 0.5, Y * freq + 0.5, perm)
+        amp *= 0.5
+        freq *= 2.0
+    P = np.stack([X, Y, Z], -1)
+    a = P[:-1, :-1].reshape(-1, 3)
+    b = P[1:, :-1].reshape(-1, 3)
+    c = P[1:, 1:].reshape(-1, 3)
+    d = P[:-1, 1:].reshape(-1, 3)
+    terr = np.empty((cells * cells * 2, 3, 3))
+    terr[0::2] = np.stack([a, b, c], 1)
+    terr[1::2] = np.stack([a, c, d], 1)
+    mats = []
+    for _ in range(n_mats):
+        mats.append(principled_bsdf(tuple(rng.uniform(0.0, 1.0, 3)),
+                                    roughness=rng.uniform(0.05, 1.0),
+                                    metallic=1.0 if rng.uniform() < 0.3 else 0.0,
+                                    ior=1.5,
+                                    clearcoat=rng.uniform(0.0, 0.5),
+                                    clearcoat_roughness=0.03))
+    light_mats = []
+    for _ in range(3):
+        e = rng.uniform(5.0, 20.0)
+        light_mats.append(principled_bsdf((0.8, 0.8, 0.8), emission=(e, e, e)))
+    mats = mats + light_mats
+    tris = [terr]
+    tri_mesh = [np.zeros(len(terr), np.uint32)]
+    mesh_mat = [0]
+    smooth = [0]
+    sv, sf = icosphere(sphere_level)
+    for i in range(n_spheres):
+        cx, cy = rng.uniform(-45.0, 45.0, 2)
+        r = rng.uniform(0.5, 2.5)
+        ix = int(np.clip((cx + 50.0) / 100.0 * cells, 0, cells))
+        iy = int(np.clip((cy + 50.0) / 100.0 * cells, 0, cells))
+        cz = Z[ix, iy] + r * 0.8
+        p = sv * r + np.array([cx, cy, cz])
+        tris.append(p[sf])
+        tri_mesh.append(np.full(len(sf), len(mesh_mat), np.uint32))
+        mesh_mat.append(int(rng.integers(0, n_mats)))
+        smooth.append(0)
+    for k in range(3):
+        cx, cy = rng.uniform(-30.0, 30.0, 2)
+        h = 18.0 + 4.0 * k
+        s = 4.0
+        q = _quad((cx - s, cy - s, h), (cx - s, cy + s, h), (cx + s, cy + s, h), (cx + s, cy - s, h))
+        tris.append(np.array(q, np.float64))
+        tri_mesh.append(np.full(2, len(mesh_mat), np.uint32))
+        mesh_mat.append(n_mats + k)
+        smooth.append(0)
+    tris = np.concatenate(tris)
+    norms = np.zeros_like(tris)
+    cam = look_camera((-70.0, -10.0, 30.0), 10.0, -22.0, 24.0)
+    return _finish("terrain_c2", tris, norms, np.concatenate(tri_mesh), mesh_mat, smooth, mats, cam,
+                   W, H, spp, max_depth)
+
+
+def material_zoo(W=64, H=64, spp=4, max_depth=5) -> Scene:
+    """Cornell-like box whose blocks and walls carry metallic, glass, clearcoat,
+    sheen, subsurface, tinted and rough/smooth Disney parameters, plus a smooth
+    sphere: exercises every BSDF lobe of the integrator."""
+    base = cornell_c0(W, H, spp, max_depth)
+    mats = list(base.mats)
+    mats.append(principled_bsdf((0.9, 0.6, 0.3), roughness=0.2, metallic=1.0))                    # 4 gold metal
+    mats.append(principled_bsdf((1.0, 1.0, 1.0), roughness=0.05, spec_trans=1.0, ior=1.33))     # 5 glass
+    mats.append(principled_bsdf((0.2, 0.3, 0.8), roughness=0.6, clearcoat=1.0,
+                                clearcoat_roughness=0.1, sheen=0.5, sheen_tint=0.5))             # 6 clearcoat
+    mats.append(principled_bsdf((0.7, 0.2, 0.2), roughness=0.9, subsurface=0.7,
+                                specular_tint=0.8))                                               # 7 subsurface
+    mesh_mat = base.mesh_mat.copy()
+    mesh_mat[6] = 4
+    mesh_mat[7] = 5
+    mesh_mat[2] = 6
+    mesh_mat[0] = 7
+    sv, sf = icosphere(2)
+    p = sv * 0.25 + np.array([0.0, 0.0, 0.5])
+    n = sv
+    verts = np.concatenate([base.verts.astype(np.float64), p[sf]])
+    norms = np.concatenate([base.norms.astype(np.float64), n[sf]])
+    tri_mesh = np.concatenate([base.tri_mesh, np.full(len(sf), len(mesh_mat), np.uint32)])
+    mats.append(principled_bsdf((0.8, 0.8, 0.8), roughness=0.3, metallic=0.5, clearcoat=0.5,
+                                clearcoat_roughness=0.2))                                         # 8 sphere
+    mesh_mat = np.concatenate([mesh_mat, [8]])
+    smooth = np.concatenate([base.mesh_smooth, [1]])
+    return _finish("material_zoo", verts, norms, tri_mesh, mesh_mat, smooth, mats, base.camera,
+                   W, H, spp, max_depth)

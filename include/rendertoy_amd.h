@@ -1,0 +1,177 @@
+/*
+ * rendertoy_amd.h -- C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * RenderToy has no FFI of its own: its "operator API" is the virtual
+ * Renderer::Render() on a RenderContext (include/RenderToy/renderer.h:44-77).
+ * These entry points are what that API binds to.  The C++ layer under
+ * include/RenderToy/ (World, RenderContext, PathTracingRenderer, ...) flattens
+ * a World into rt_scene_desc once per RenderContext and calls them from
+ * Render().  Plain pointers and sizes only; no C++ or HIP types cross the ABI.
+ *
+ * Errors: every entry point returns RT_OK (0) or an RT_ERR_* code and never
+ * lets a C++ exception escape; rt_last_error() returns a thread-local message
+ * for the last failure on the calling thread.  The hot path of the reference
+ * has no error reporting at all (src/renderer.cpp:236-427).
+ *
+ * Threading: calls on one rt_scene are serialised by the caller (the
+ * reference's Render() is not re-entrant either, it shares one global RNG,
+ * src/rtmath.cpp:355).  Distinct scenes may be used from distinct threads.
+ * All randomness is counter-based on (seed, pixel, sample, depth), so results
+ * do not depend on launch shape or thread count.
+ */
+#ifndef RENDERTOY_AMD_H
+#define RENDERTOY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_ABI_VERSION 1
+
+enum rt_status {
+    RT_OK = 0,
+    RT_ERR_INVALID = 1,     /* bad argument or scene description */
+    RT_ERR_HIP = 2,         /* HIP runtime failure (message names the call) */
+    RT_ERR_NO_DEVICE = 3,   /* no gfx950 device visible */
+    RT_ERR_OOM = 4,
+    RT_ERR_INTERNAL = 5
+};
+
+/* One material, field for field PrincipledBSDF (include/RenderToy/material.h:12-77)
+ * after its constructor's roughness clamp (src/material.cpp:29-32).  96 bytes. */
+typedef struct rt_material {
+    float base_color[3];
+    float emission[3];
+    float roughness, metallic, spec_trans, ior, anisotropic, subsurface, specular_tint, sheen, sheen_tint,
+        clearcoat, clearcoat_roughness, at_distance;
+    float extinction[3];
+    float reserved[3];
+} rt_material;
+
+/* A World after assembly (include/RenderToy/world.h:13-41). */
+typedef struct rt_scene_desc {
+    uint32_t num_triangles;
+    const float *vertices;        /* [num_triangles][3][3] Triangle::vert_w (object.cpp:224-237) */
+    const float *normals;         /* [num_triangles][3][3] Triangle::norm, or NULL when no mesh is smooth */
+    const uint32_t *tri_mesh;     /* [num_triangles] owning Mesh index (Triangle::parent) */
+    uint32_t num_meshes;
+    const uint32_t *mesh_material;/* [num_meshes] index into materials (Mesh::tex) */
+    const uint8_t *mesh_smooth;   /* [num_meshes] Mesh::smooth */
+    uint32_t num_materials;
+    const rt_material *materials;
+    uint32_t num_emitters;
+    const uint32_t *emitters;     /* triangle indices in World::emissive_triangles order (world.cpp:24-37) */
+    float sky_emission[3];        /* World::sky_emission (world.h:34) */
+    float ground_reflection[3];   /* World::ground_reflection (world.h:35) */
+} rt_scene_desc;
+
+/* The camera as Renderer::PrepareScreenSpace leaves it (src/renderer.cpp:103-124):
+ * Camera::object_to_world row-major, the screen half extents, the clip planes
+ * and the film size.  Rays go through pixel corners, y = 0 is the bottom row
+ * (src/renderer.cpp:253-259). */
+typedef struct rt_camera {
+    float o2w[16];
+    float top, right;
+    float near_clip, far_clip;
+    uint32_t width, height;
+} rt_camera;
+
+typedef struct rt_scene rt_scene;
+
+/* Flattens the World, builds the BVH4 on the host and uploads it to the
+ * current HIP device.  Replaces RenderContext::RenderContext's
+ * `bvh = new BVH(world->triangles)` (src/renderer.cpp:77-85, src/bvh.cpp:242-264). */
+int rt_scene_create(const rt_scene_desc *desc, rt_scene **out);
+
+/* Frees every device allocation of the scene.  Replaces RenderContext::~RenderContext
+ * (src/renderer.cpp:87-91). */
+int rt_scene_destroy(rt_scene *scene);
+
+/* AOV pass selection bits for rt_render_aov. */
+enum rt_aov_pass {
+    RT_AOV_ID = 1u << 0,      /* hit triangle index, 0xFFFFFFFF on miss (u32) */
+    RT_AOV_T = 1u << 1,       /* hit distance, 1e7 on miss (f32) */
+    RT_AOV_NORMAL = 1u << 2,  /* NormalRenderer value (n+1)/2, 0 on miss (3 x f32) */
+    RT_AOV_DEPTH = 1u << 3,   /* DepthBufferRenderer value, 1 on miss (f32) */
+    RT_AOV_ALBEDO = 1u << 4,  /* AlbedoRenderer value, 0 on miss (3 x f32) */
+    RT_AOV_MASK = 1u << 5     /* IntersectTestRenderer value, 1 hit / 0 miss (f32) */
+};
+
+/* Host destinations for rt_render_aov; any pointer may be NULL (not copied).
+ * Every array holds width*height entries in RenderContext::buffer order. */
+typedef struct rt_aov_out {
+    uint32_t *tri_id;
+    float *t;
+    float *normal;
+    float *depth;
+    float *albedo;
+    float *mask;
+} rt_aov_out;
+
+/* Primary-ray passes of NormalRenderer::Render (renderer.cpp:186-229),
+ * DepthBufferRenderer::Render (renderer.cpp:138-174), AlbedoRenderer::Render
+ * (renderer.cpp:391-427) and IntersectTestRenderer::Render (renderer.cpp:20-54),
+ * all selected passes in one launch.  Results stay on the device; when `out`
+ * is not NULL the selected passes are also copied to the host pointers it
+ * names.  Blocks until done. */
+int rt_render_aov(rt_scene *scene, const rt_camera *cam, uint32_t passes, const rt_aov_out *out);
+
+/* Path tracer parameters.  max_depth counts path vertices; the reference
+ * hardwires 5 (`depth > 4`, src/renderer.cpp:274). */
+typedef struct rt_pt_params {
+    uint32_t spp;
+    uint32_t max_depth;
+    uint64_t seed;
+    uint32_t batch_paths;   /* paths in flight per wavefront batch; 0 = automatic */
+    uint32_t flags;         /* RT_PT_* */
+} rt_pt_params;
+
+enum rt_pt_flags {
+    RT_PT_NO_GRAPH = 1u << 0,    /* launch the bounce loop kernel by kernel instead of replaying a hipGraph */
+    RT_PT_NO_SORT = 1u << 1      /* keep extension rays in path order (no coherence sort) */
+};
+
+/* PathTracingRenderer::Render (src/renderer.cpp:236-270) with Radiance and
+ * DirectLight (renderer.cpp:272-384): buffer += L/spp per sample, in sample
+ * order per pixel.  The frame stays on the device; when rgb_host is not NULL
+ * it is copied there as float[width*height][3].  Blocks until done. */
+int rt_render_pt(rt_scene *scene, const rt_camera *cam, const rt_pt_params *params, float *rgb_host);
+
+/* Counters and timings of the last render call on this scene. */
+typedef struct rt_stats {
+    double total_ms;              /* device time of the whole call (HIP events) */
+    double kernel_ms;             /* device time of the dominant kernel(s) */
+    uint64_t rays;                /* rays traced: primary (once per pixel) + extension + shadow */
+    uint64_t samples;             /* camera samples */
+    uint64_t node_visits;         /* BVH4 node fetches (only when counting is on) */
+    uint64_t tri_tests;           /* triangle tests (only when counting is on) */
+    uint32_t bvh_nodes, bvh_depth, bvh_max_stack, reserved;
+    double build_ms;              /* host BVH build time at rt_scene_create */
+    uint64_t shadow_rays;         /* shadow rays among `rays` */
+    uint64_t primary_rays;        /* primary rays among `rays` */
+} rt_stats;
+
+int rt_scene_stats(const rt_scene *scene, rt_stats *out);
+
+/* Turns per-ray node/triangle counting on or off for later calls (slower). */
+int rt_scene_set_counting(rt_scene *scene, int on);
+
+/* Device pointer of a pass left on the device by the last render, for callers
+ * that keep results in HBM (pass = one RT_AOV_* bit, or 0 for the path-traced
+ * RGB frame). */
+int rt_scene_device_output(rt_scene *scene, uint32_t pass, void **dev_ptr, size_t *bytes);
+
+/* Thread-local description of the last error. */
+const char *rt_last_error(void);
+
+/* Name and compute-unit count of the device the library runs on. */
+int rt_device_info(char *name, size_t name_len, int *compute_units);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -59,7 +59,7 @@ class Oracle:
         L.orc_philox4x32_10.argtypes = [vp, vp, vp]
         L.orc_render_pt.argtypes = [vp, C.c_int, vp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
-                                    C.c_uint64, C.c_uint64, vp, C.c_int]
+                                    C.c_uint64, C.c_uint64, vp, C.c_int, vp]
         L.orc_render_pt.restype = C.c_int64
         L.orc_pt_samples.argtypes = [vp, C.c_int, vp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_uint64,
                                      vp, vp, C.c_uint32, C.c_uint64, vp, C.c_int]
@@ -179,10 +179,11 @@ class OracleScene:
         x0, y0, x1, y1 = rect or (0, 0, W, H)
         o2w, top, right = self._cam(W, H)
         out = np.zeros(((y1 - y0) * (x1 - x0), 3), np.float32)
-        rays = self.orc.lib.orc_render_pt(self.h, mode, _ptr(o2w), float(top), float(right), W, H, x0, y0, x1, y1,
-                                          spp, max_depth, 0 if rng == "mt" else 1, seed, warmup, _ptr(out),
-                                          1 if rng == "mt" else threads)
-        return out, rays
+        counts = np.zeros(2, np.int64)
+        self.orc.lib.orc_render_pt(self.h, mode, _ptr(o2w), float(top), float(right), W, H, x0, y0, x1, y1,
+                                   spp, max_depth, 0 if rng == "mt" else 1, seed, warmup, _ptr(out),
+                                   1 if rng == "mt" else threads, _ptr(counts))
+        return out, {"extension": int(counts[0]), "shadow": int(counts[1])}
 
     def pt_samples(self, pixel, sample, W=None, H=None, max_depth=None, seed=0, mode=0, threads=8):
         W = W or self.sc.width

@@ -719,7 +719,8 @@ struct PT {
     orc_scene *s;
     int mode;
     uint32_t max_depth;
-    int64_t rays = 0;
+    int64_t rays = 0;          /* extension rays traced */
+    int64_t shadow = 0;        /* shadow rays whose contribution is not identically zero */
 
     /* DirectLight (renderer.cpp:335-384), World::SampleEmitter (world.cpp:5-17) */
     V3 direct_light(const RayState &st, V3 ray_dir, uint32_t tri, V3 pos, Rng &rng) {
@@ -736,7 +737,14 @@ struct PT {
         V3 emit_pos = add(add(muls(et.e1, a), muls(et.e2, b)), et.v[0]);
         V3 dir_to = normalized(sub(emit_pos, pos));
         Hit sh = intersect(s, mode, pos, dir_to, tri);
-        ++rays;
+        {
+            /* counting only: Eval has no side effects */
+            float pdf_c, lp_c;
+            V3 f_c = BsdfEval(mat_of(s, s->tris[tri]), st.eta, neg(ray_dir), st.ffnormal, dir_to, pdf_c);
+            V3 e_c = get_emission(s, et, 0.0f, 0.0f, emit_pos, pos, neg(dir_to), lp_c);
+            V3 c_c = muls(mulv(muls(f_c, PowerHeuristic(lp_c, pdf_c)), e_c), 1.0f);
+            if (pdf_c > 0.0f && !(c_c.x == 0.0f && c_c.y == 0.0f && c_c.z == 0.0f)) ++shadow;
+        }
         if (sh.id == kNone || sh.id == emit) {
             float su = sh.id == kNone ? 0.0f : sh.u, sv = sh.id == kNone ? 0.0f : sh.v;
             float lightpdf;
@@ -960,8 +968,9 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 /* PathTracingRenderer::Render (renderer.cpp:236-270) */
 int64_t orc_render_pt(orc_scene *s, int mode, const float *o2w, float top, float right, uint32_t W, uint32_t H,
                       uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t spp, uint32_t max_depth,
-                      int rng_mode, uint64_t seed, uint64_t warmup, float *out, int nthreads) {
+                      int rng_mode, uint64_t seed, uint64_t warmup, float *out, int nthreads, int64_t *counts) {
     if (mode == 1) ensure_octree(s);
+    int64_t total_shadow = 0;
     float divW = 1.0f / float(W), divH = 1.0f / float(H);
     uint32_t RW = x1 - x0, RH = y1 - y0;
     std::memset(out, 0, sizeof(float) * 3 * (size_t)RW * RH);
@@ -986,10 +995,11 @@ int64_t orc_render_pt(orc_scene *s, int mode, const float *o2w, float top, float
                     cur = add(cur, divs(L, (float)spp));
                     out[q * 3] = cur.x; out[q * 3 + 1] = cur.y; out[q * 3 + 2] = cur.z;
                 }
-        return pt.rays;
+        if (counts) { counts[0] = pt.rays; counts[1] = pt.shadow; }
+        return pt.rays + pt.shadow;
     }
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total_rays) num_threads(nthreads > 0 ? nthreads : 1)
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : total_rays, total_shadow) num_threads(nthreads > 0 ? nthreads : 1)
     for (int64_t q = 0; q < (int64_t)RW * RH; ++q) {
         uint32_t x = x0 + (uint32_t)(q % RW), y = y0 + (uint32_t)(q / RW);
         PT pt{s, mode, max_depth};
@@ -1008,8 +1018,10 @@ int64_t orc_render_pt(orc_scene *s, int mode, const float *o2w, float top, float
         }
         out[q * 3] = acc.x; out[q * 3 + 1] = acc.y; out[q * 3 + 2] = acc.z;
         total_rays += pt.rays;
+        total_shadow += pt.shadow;
     }
-    return total_rays;
+    if (counts) { counts[0] = total_rays; counts[1] = total_shadow; }
+    return total_rays + total_shadow;
 }
 
 void orc_pt_samples(orc_scene *s, int mode, const float *o2w, float top, float right, uint32_t W, uint32_t H,

@@ -102,11 +102,12 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * [(y1-y0)*(x1-x0)][3], accumulated as buffer += L/spp in sample order.
  * max_depth = number of path vertices (the reference hardwires 5).
  * nthreads > 1 is allowed only with rng_mode 1.
- * Returns total rays traced (extension + shadow). */
+ * counts (may be NULL) receives {extension rays, shadow rays whose contribution
+ * is not identically zero}; returns their sum. */
 int64_t orc_render_pt(orc_scene *s, int mode, const float *o2w, float top, float right,
                       uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1,
                       uint32_t spp, uint32_t max_depth, int rng_mode, uint64_t seed,
-                      uint64_t warmup, float *out, int nthreads);
+                      uint64_t warmup, float *out, int nthreads, int64_t *counts);
 
 /* Per-sample radiance for listed (pixel, sample) pairs in Philox mode, for
  * per-path comparisons: out[n][3]. */

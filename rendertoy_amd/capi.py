@@ -1,0 +1,199 @@
+"""Python binding of the C ABI in include/rendertoy_amd.h (ctypes).
+
+This is the binding a Python caller of RenderToy's renderers would use; the
+C++ binding is include/RenderToy/.  It loads the in-tree
+``rendertoy_amd/lib/librendertoy_amd.so`` and raises if it is missing: there
+is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "librendertoy_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "rendertoy_amd.h")
+
+RT_OK = 0
+RT_ERR = {1: "RT_ERR_INVALID", 2: "RT_ERR_HIP", 3: "RT_ERR_NO_DEVICE", 4: "RT_ERR_OOM", 5: "RT_ERR_INTERNAL"}
+AOV_ID, AOV_T, AOV_NORMAL, AOV_DEPTH, AOV_ALBEDO, AOV_MASK = 1, 2, 4, 8, 16, 32
+AOV_ALL = 63
+PT_NO_GRAPH, PT_NO_SORT = 1, 2
+NONE = 0xFFFFFFFF
+
+
+class RtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (RT_ERR.get(code, str(code)), msg))
+        self.code = code
+
+
+class rt_material(C.Structure):
+    _fields_ = [("data", C.c_float * 24)]
+
+
+class rt_scene_desc(C.Structure):
+    _fields_ = [("num_triangles", C.c_uint32), ("vertices", C.c_void_p), ("normals", C.c_void_p),
+                ("tri_mesh", C.c_void_p), ("num_meshes", C.c_uint32), ("mesh_material", C.c_void_p),
+                ("mesh_smooth", C.c_void_p), ("num_materials", C.c_uint32), ("materials", C.c_void_p),
+                ("num_emitters", C.c_uint32), ("emitters", C.c_void_p), ("sky_emission", C.c_float * 3),
+                ("ground_reflection", C.c_float * 3)]
+
+
+class rt_camera(C.Structure):
+    _fields_ = [("o2w", C.c_float * 16), ("top", C.c_float), ("right", C.c_float), ("near_clip", C.c_float),
+                ("far_clip", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class rt_aov_out(C.Structure):
+    _fields_ = [("tri_id", C.c_void_p), ("t", C.c_void_p), ("normal", C.c_void_p), ("depth", C.c_void_p),
+                ("albedo", C.c_void_p), ("mask", C.c_void_p)]
+
+
+class rt_pt_params(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("batch_paths", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+class rt_stats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double), ("rays", C.c_uint64),
+                ("samples", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_uint32), ("bvh_max_stack", C.c_uint32),
+                ("reserved", C.c_uint32), ("build_ms", C.c_double), ("shadow_rays", C.c_uint64),
+                ("primary_rays", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+
+
+def header_functions(path=HEADER):
+    """Names of the functions include/rendertoy_amd.h declares."""
+    txt = open(path).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char|void)\s*\**\s*(rt_\w+)\s*\(", txt, flags=re.M)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("librendertoy_amd.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.rt_last_error.restype = C.c_char_p
+    L.rt_scene_create.argtypes = [C.POINTER(rt_scene_desc), C.POINTER(vp)]
+    L.rt_scene_destroy.argtypes = [vp]
+    L.rt_render_aov.argtypes = [vp, C.POINTER(rt_camera), C.c_uint32, C.POINTER(rt_aov_out)]
+    L.rt_render_pt.argtypes = [vp, C.POINTER(rt_camera), C.POINTER(rt_pt_params), vp]
+    L.rt_scene_stats.argtypes = [vp, C.POINTER(rt_stats)]
+    L.rt_scene_set_counting.argtypes = [vp, C.c_int]
+    L.rt_scene_device_output.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_size_t)]
+    L.rt_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
+    _lib = L
+    return L
+
+
+def check(code):
+    if code != RT_OK:
+        raise RtError(code, lib().rt_last_error().decode(errors="replace"))
+
+
+def device_info():
+    buf = C.create_string_buffer(64)
+    cu = C.c_int()
+    check(lib().rt_device_info(buf, 64, C.byref(cu)))
+    return buf.value.decode(), cu.value
+
+
+def make_camera(cam, W, H) -> rt_camera:
+    """rt_camera from a scenes.Camera: PrepareScreenSpace (renderer.cpp:103-124)."""
+    top, right = cam.screen_space(W, H)
+    c = rt_camera()
+    for i in range(16):
+        c.o2w[i] = float(np.float32(cam.o2w[i]))
+    c.top, c.right = float(top), float(right)
+    c.near_clip, c.far_clip = float(np.float32(cam.near)), float(np.float32(cam.far))
+    c.width, c.height = W, H
+    return c
+
+
+class Scene:
+    """An rt_scene built from a scenes.Scene record."""
+
+    def __init__(self, sc):
+        L = lib()
+        self.sc = sc
+        f32 = lambda a: np.ascontiguousarray(a, np.float32)
+        u32 = lambda a: np.ascontiguousarray(a, np.uint32)
+        self._keep = k = dict(v=f32(sc.verts), n=f32(sc.norms), tm=u32(sc.tri_mesh), mm=u32(sc.mesh_mat),
+                              ms=np.ascontiguousarray(sc.mesh_smooth, np.uint8), mats=f32(sc.mats),
+                              em=u32(sc.emitters))
+        d = rt_scene_desc()
+        d.num_triangles = len(k["v"])
+        d.vertices = k["v"].ctypes.data
+        d.normals = k["n"].ctypes.data if np.any(k["ms"]) else None
+        d.tri_mesh = k["tm"].ctypes.data
+        d.num_meshes = len(k["mm"])
+        d.mesh_material = k["mm"].ctypes.data
+        d.mesh_smooth = k["ms"].ctypes.data
+        d.num_materials = len(k["mats"])
+        d.materials = k["mats"].ctypes.data
+        d.num_emitters = len(k["em"])
+        d.emitters = k["em"].ctypes.data if len(k["em"]) else None
+        for i in range(3):
+            d.sky_emission[i] = float(sc.sky[i])
+            d.ground_reflection[i] = float(sc.ground[i])
+        h = C.c_void_p()
+        check(L.rt_scene_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self._keep = None   # the library copied everything it needs
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def camera(self, W=None, H=None):
+        return make_camera(self.sc.camera, W or self.sc.width, H or self.sc.height)
+
+    def set_counting(self, on=True):
+        check(lib().rt_scene_set_counting(self.h, 1 if on else 0))
+
+    def stats(self) -> dict:
+        s = rt_stats()
+        check(lib().rt_scene_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def render_aov(self, W=None, H=None, passes=AOV_ALL, download=True):
+        cam = self.camera(W, H)
+        n = cam.width * cam.height
+        out = None
+        res = {}
+        if download:
+            res = dict(id=np.zeros(n, np.uint32), t=np.zeros(n, np.float32), normal=np.zeros((n, 3), np.float32),
+                       depth=np.zeros(n, np.float32), albedo=np.zeros((n, 3), np.float32),
+                       mask=np.zeros(n, np.float32))
+            out = rt_aov_out(res["id"].ctypes.data, res["t"].ctypes.data, res["normal"].ctypes.data,
+                             res["depth"].ctypes.data, res["albedo"].ctypes.data, res["mask"].ctypes.data)
+        check(lib().rt_render_aov(self.h, C.byref(cam), passes, C.byref(out) if out is not None else None))
+        return res
+
+    def render_pt(self, W=None, H=None, spp=None, max_depth=None, seed=0, batch_paths=0, flags=0, download=True):
+        cam = self.camera(W, H)
+        p = rt_pt_params(spp or self.sc.spp, max_depth or self.sc.max_depth, seed, batch_paths, flags)
+        rgb = np.zeros((cam.width * cam.height, 3), np.float32) if download else None
+        check(lib().rt_render_pt(self.h, C.byref(cam), C.byref(p), rgb.ctypes.data if download else None))
+        return rgb

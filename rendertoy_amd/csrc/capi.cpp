@@ -1,0 +1,421 @@
+// capi.cpp -- the extern "C" boundary: scene flattening and upload, AOV passes.
+//
+// Built with -ffp-contract=off like the kernels: the triangle cache below is
+// computed in the same single-precision operation order as
+// Triangle::UpdateCache (src/object.cpp:224-243).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bvh_build.h"
+#include "pt.h"
+#include "scene.h"
+
+namespace rtamd {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+namespace {
+
+struct HipError {
+    hipError_t e;
+    const char *what;
+};
+
+#define RT_HIP(call)                                                  \
+    do {                                                              \
+        hipError_t _e = (call);                                       \
+        if (_e != hipSuccess) throw HipError{_e, #call};              \
+    } while (0)
+
+struct InvalidArg {
+    std::string msg;
+};
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const HipError &h) {
+        set_error(std::string("HIP error in ") + h.what + ": " + hipGetErrorString(h.e));
+        return h.e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+    } catch (const InvalidArg &a) {
+        set_error(a.msg);
+        return RT_ERR_INVALID;
+    } catch (const std::bad_alloc &) {
+        set_error("host out of memory");
+        return RT_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return RT_ERR_INTERNAL;
+    } catch (...) {
+        set_error("unknown error");
+        return RT_ERR_INTERNAL;
+    }
+}
+
+void *dev_upload(rt_scene *s, const void *host, size_t bytes) {
+    DevBuf b;
+    b.bytes = bytes ? bytes : 16;
+    RT_HIP(hipMalloc(&b.p, b.bytes));
+    s->owned.push_back(b);
+    if (host && bytes) RT_HIP(hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice));
+    return b.p;
+}
+
+// Vector3f operations in the reference's order (rtmath.h:126-150,320-329,399-418).
+struct H3 {
+    float x, y, z;
+};
+inline H3 hsub(H3 a, H3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline H3 hcross(H3 a, H3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float hlen(H3 a) {
+    float r = 0.0f;
+    r += a.x * a.x;
+    r += a.y * a.y;
+    r += a.z * a.z;
+    return std::sqrt(r);
+}
+inline H3 hnorm(H3 a) {
+    float l = hlen(a);
+    return {a.x / l, a.y / l, a.z / l};
+}
+
+void check_device(rt_scene *s) {
+    int dev = 0, n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw InvalidArg{"no HIP device visible"};
+    RT_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    RT_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        throw InvalidArg{std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950 only"};
+    s->device = dev;
+    s->cu_count = prop.multiProcessorCount;
+}
+
+void ensure_ovf(rt_scene *s, uint32_t threads) {
+    uint32_t deep = s->max_stack + 1 > (uint32_t)kLdsStack ? s->max_stack + 1 - kLdsStack : 1;
+    size_t need = (size_t)deep * threads * sizeof(uint32_t);
+    if (need <= s->ovf_bytes) return;
+    if (s->d_ovf_ref) RT_HIP(hipFree(s->d_ovf_ref));
+    if (s->d_ovf_t) RT_HIP(hipFree(s->d_ovf_t));
+    s->d_ovf_ref = nullptr;
+    s->d_ovf_t = nullptr;
+    RT_HIP(hipMalloc(&s->d_ovf_ref, need));
+    RT_HIP(hipMalloc(&s->d_ovf_t, need));
+    s->ovf_bytes = need;
+}
+
+void validate_camera(const rt_camera *cam) {
+    if (!cam) throw InvalidArg{"camera is NULL"};
+    if (cam->width == 0 || cam->height == 0) throw InvalidArg{"camera has zero width or height"};
+    if ((uint64_t)cam->width * cam->height >= (1ull << 31)) throw InvalidArg{"frame larger than 2^31 pixels"};
+}
+
+}  // namespace
+
+// Shared with pt.cpp.
+uint32_t persistent_grid(rt_scene *s, int blocks_per_cu) { return (uint32_t)(blocks_per_cu * s->cu_count); }
+void ensure_overflow(rt_scene *s, uint32_t threads) { ensure_ovf(s, threads); }
+int run_guarded_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        set_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+    }
+    return RT_OK;
+}
+
+}  // namespace rtamd
+
+using namespace rtamd;
+
+rt_scene::~rt_scene() {
+    if (pt) pt_destroy(pt);
+    for (auto &b : owned) (void)hipFree(b.p);
+    for (auto &b : aov_bufs) (void)hipFree(b.p);
+    if (d_counters) (void)hipFree(d_counters);
+    if (d_stat) (void)hipFree(d_stat);
+    if (d_ovf_ref) (void)hipFree(d_ovf_ref);
+    if (d_ovf_t) (void)hipFree(d_ovf_t);
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+const char *rt_last_error(void) { return g_err.c_str(); }
+
+int rt_device_info(char *name, size_t name_len, int *compute_units) {
+    return guarded([&]() {
+        int dev = 0, n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            set_error("no HIP device visible");
+            return (int)RT_ERR_NO_DEVICE;
+        }
+        RT_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        RT_HIP(hipGetDeviceProperties(&prop, dev));
+        if (name && name_len) {
+            std::snprintf(name, name_len, "%s", prop.gcnArchName);
+        }
+        if (compute_units) *compute_units = prop.multiProcessorCount;
+        return (int)RT_OK;
+    });
+}
+
+int rt_scene_create(const rt_scene_desc *desc, rt_scene **out) {
+    if (!out) {
+        set_error("out is NULL");
+        return RT_ERR_INVALID;
+    }
+    *out = nullptr;
+    rt_scene *s = new (std::nothrow) rt_scene();
+    if (!s) {
+        set_error("host out of memory");
+        return RT_ERR_OOM;
+    }
+    int rc = guarded([&]() {
+        if (!desc) throw InvalidArg{"scene description is NULL"};
+        const uint32_t n = desc->num_triangles;
+        if (n && (!desc->vertices || !desc->tri_mesh)) throw InvalidArg{"vertices/tri_mesh missing"};
+        if (desc->num_meshes && (!desc->mesh_material || !desc->mesh_smooth))
+            throw InvalidArg{"mesh_material/mesh_smooth missing"};
+        if (desc->num_materials && !desc->materials) throw InvalidArg{"materials missing"};
+        if (desc->num_emitters && !desc->emitters) throw InvalidArg{"emitters missing"};
+        bool any_smooth = false;
+        for (uint32_t m = 0; m < desc->num_meshes; ++m) {
+            if (desc->mesh_material[m] >= desc->num_materials) throw InvalidArg{"mesh material index out of range"};
+            any_smooth |= desc->mesh_smooth[m] != 0;
+        }
+        for (uint32_t i = 0; i < n; ++i)
+            if (desc->tri_mesh[i] >= desc->num_meshes) throw InvalidArg{"triangle mesh index out of range"};
+        for (uint32_t e = 0; e < desc->num_emitters; ++e)
+            if (desc->emitters[e] >= n) throw InvalidArg{"emitter triangle index out of range"};
+        if (any_smooth && n && !desc->normals) throw InvalidArg{"smooth mesh without vertex normals"};
+        check_device(s);
+        RT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (auto &e : s->ev) RT_HIP(hipEventCreate(&e));
+
+        // Triangle::UpdateCache (src/object.cpp:224-243)
+        std::vector<float> cache((size_t)n * 9);
+        std::vector<float4> geo(n ? n : 1);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float *v = desc->vertices + 9 * (size_t)i;
+            H3 v0{v[0], v[1], v[2]}, v1{v[3], v[4], v[5]}, v2{v[6], v[7], v[8]};
+            H3 e1 = hsub(v1, v0), e2 = hsub(v2, v0);
+            float area = hlen(hcross(e1, e2)) * 0.5f;
+            H3 nrm = hnorm(hcross(hnorm(e1), hsub(v2, v1)));
+            float *c = &cache[9 * (size_t)i];
+            c[0] = v0.x; c[1] = v0.y; c[2] = v0.z;
+            c[3] = e1.x; c[4] = e1.y; c[5] = e1.z;
+            c[6] = e2.x; c[7] = e2.y; c[8] = e2.z;
+            geo[i] = make_float4(nrm.x, nrm.y, nrm.z, area);
+        }
+        Bvh4 bvh = build_bvh4(n, cache.data());
+        std::vector<float4> tris((size_t)3 * (n ? n : 1));
+        for (uint32_t k = 0; k < n; ++k) {
+            uint32_t i = bvh.order[k];
+            const float *c = &cache[9 * (size_t)i];
+            uint32_t id = i;
+            float idf;
+            std::memcpy(&idf, &id, 4);
+            tris[3 * (size_t)k + 0] = make_float4(c[0], c[1], c[2], c[3]);
+            tris[3 * (size_t)k + 1] = make_float4(c[4], c[5], c[6], c[7]);
+            tris[3 * (size_t)k + 2] = make_float4(c[8], idf, 0.0f, 0.0f);
+        }
+        std::vector<uint32_t> info(n ? n : 1);
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t m = desc->tri_mesh[i];
+            info[i] = desc->mesh_material[m] | (desc->mesh_smooth[m] ? 0x80000000u : 0u);
+        }
+        std::vector<EmitterRec> emit(desc->num_emitters ? desc->num_emitters : 1);
+        for (uint32_t e = 0; e < desc->num_emitters; ++e) {
+            uint32_t i = desc->emitters[e];
+            const float *c = &cache[9 * (size_t)i];
+            EmitterRec &r = emit[e];
+            std::memset(&r, 0, sizeof(r));
+            for (int k = 0; k < 3; ++k) {
+                r.v0[k] = c[k];
+                r.e1[k] = c[3 + k];
+                r.e2[k] = c[6 + k];
+            }
+            r.normal[0] = geo[i].x;
+            r.normal[1] = geo[i].y;
+            r.normal[2] = geo[i].z;
+            r.area = geo[i].w;
+            r.tri = i;
+        }
+        DevScene &ds = s->ds;
+        ds.nodes = (const Node4 *)dev_upload(s, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4));
+        ds.tris = (const float4 *)dev_upload(s, tris.data(), tris.size() * sizeof(float4));
+        ds.geo = (const float4 *)dev_upload(s, geo.data(), geo.size() * sizeof(float4));
+        ds.info = (const uint32_t *)dev_upload(s, info.data(), info.size() * sizeof(uint32_t));
+        ds.vnorm = any_smooth ? (const float *)dev_upload(s, desc->normals, (size_t)n * 9 * sizeof(float)) : nullptr;
+        ds.mats = (const rt_material *)dev_upload(s, desc->materials,
+                                                  (size_t)desc->num_materials * sizeof(rt_material));
+        ds.emit = (const EmitterRec *)dev_upload(s, emit.data(), emit.size() * sizeof(EmitterRec));
+        ds.num_emit = desc->num_emitters;
+        ds.num_tris = n;
+        for (int k = 0; k < 3; ++k) {
+            ds.sky[k] = desc->sky_emission[k];
+            ds.ground[k] = desc->ground_reflection[k];
+        }
+        s->num_tris = n;
+        s->bvh_nodes = (uint32_t)bvh.nodes.size();
+        s->bvh_depth = bvh.depth;
+        s->max_stack = bvh.max_stack;
+        s->build_ms = bvh.ms;
+        RT_HIP(hipMalloc(&s->d_counters, 8 * 32 * sizeof(uint32_t)));
+        RT_HIP(hipMalloc(&s->d_stat, 8 * sizeof(unsigned long long)));
+        s->stats.build_ms = bvh.ms;
+        s->stats.bvh_nodes = s->bvh_nodes;
+        s->stats.bvh_depth = s->bvh_depth;
+        s->stats.bvh_max_stack = s->max_stack;
+        return (int)RT_OK;
+    });
+    if (rc != RT_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return RT_OK;
+}
+
+int rt_scene_destroy(rt_scene *scene) {
+    return guarded([&]() {
+        if (scene) {
+            (void)hipStreamSynchronize(scene->stream);
+            delete scene;
+        }
+        return (int)RT_OK;
+    });
+}
+
+int rt_scene_set_counting(rt_scene *scene, int on) {
+    if (!scene) {
+        set_error("scene is NULL");
+        return RT_ERR_INVALID;
+    }
+    scene->counting = on != 0;
+    return RT_OK;
+}
+
+int rt_scene_stats(const rt_scene *scene, rt_stats *out) {
+    if (!scene || !out) {
+        set_error("scene or out is NULL");
+        return RT_ERR_INVALID;
+    }
+    *out = scene->stats;
+    return RT_OK;
+}
+
+int rt_render_aov(rt_scene *s, const rt_camera *cam, uint32_t passes, const rt_aov_out *out) {
+    return guarded([&]() {
+        if (!s) throw InvalidArg{"scene is NULL"};
+        validate_camera(cam);
+        if (passes == 0 || (passes & ~0x3Fu)) throw InvalidArg{"passes must be a non-empty set of RT_AOV_* bits"};
+        RT_HIP(hipSetDevice(s->device));
+        const uint32_t W = cam->width, H = cam->height;
+        const size_t np = (size_t)W * H;
+        if (s->aov_w != W || s->aov_h != H) {
+            for (auto &b : s->aov_bufs) RT_HIP(hipFree(b.p));
+            s->aov_bufs.clear();
+            auto mk = [&](size_t bytes) {
+                DevBuf b;
+                b.bytes = bytes;
+                RT_HIP(hipMalloc(&b.p, bytes));
+                s->aov_bufs.push_back(b);
+                return b.p;
+            };
+            s->aov.id = (uint32_t *)mk(np * 4);
+            s->aov.t = (float *)mk(np * 4);
+            s->aov.normal = (float *)mk(np * 12);
+            s->aov.depth = (float *)mk(np * 4);
+            s->aov.albedo = (float *)mk(np * 12);
+            s->aov.mask = (float *)mk(np * 4);
+            s->aov_w = W;
+            s->aov_h = H;
+        }
+        const bool count = s->counting;
+        const uint32_t grid = persistent_grid(s, aov_blocks_per_cu(count));
+        ensure_ovf(s, grid * 256);
+        const int64_t ovf_stride = (int64_t)grid * 256;
+        RT_HIP(launch_reset(s->d_counters, 8 * 32, s->stream));
+        RT_HIP(launch_reset((uint32_t *)s->d_stat, 16, s->stream));
+        RT_HIP(hipEventRecord(s->ev[0], s->stream));
+        RT_HIP(launch_aov(s->ds, *cam, s->aov, passes, s->d_counters, grid, s->d_ovf_ref, s->d_ovf_t, ovf_stride,
+                          s->d_stat, count, s->stream));
+        RT_HIP(hipEventRecord(s->ev[1], s->stream));
+        RT_HIP(hipStreamSynchronize(s->stream));
+        float ms = 0.0f;
+        RT_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+        rt_stats &st = s->stats;
+        st.kernel_ms = ms;
+        st.total_ms = ms;
+        st.rays = np;
+        st.samples = np;
+        st.shadow_rays = 0;
+        st.primary_rays = np;
+        if (count) {
+            unsigned long long c[2];
+            RT_HIP(hipMemcpy(c, s->d_stat, sizeof(c), hipMemcpyDeviceToHost));
+            st.node_visits = c[0];
+            st.tri_tests = c[1];
+        } else {
+            st.node_visits = st.tri_tests = 0;
+        }
+        if (out) {
+            auto cp = [&](void *dst, const void *src, size_t bytes) {
+                if (dst) RT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+            };
+            if (passes & RT_AOV_ID) cp(out->tri_id, s->aov.id, np * 4);
+            if (passes & RT_AOV_T) cp(out->t, s->aov.t, np * 4);
+            if (passes & RT_AOV_NORMAL) cp(out->normal, s->aov.normal, np * 12);
+            if (passes & RT_AOV_DEPTH) cp(out->depth, s->aov.depth, np * 4);
+            if (passes & RT_AOV_ALBEDO) cp(out->albedo, s->aov.albedo, np * 12);
+            if (passes & RT_AOV_MASK) cp(out->mask, s->aov.mask, np * 4);
+        }
+        return (int)RT_OK;
+    });
+}
+
+int rt_scene_device_output(rt_scene *s, uint32_t pass, void **dev_ptr, size_t *bytes) {
+    if (!s || !dev_ptr) {
+        set_error("scene or dev_ptr is NULL");
+        return RT_ERR_INVALID;
+    }
+    const size_t np = (size_t)s->aov_w * s->aov_h;
+    void *p = nullptr;
+    size_t b = 0;
+    switch (pass) {
+    case 0:
+        return pt_device_output(s, dev_ptr, bytes);
+    case RT_AOV_ID: p = s->aov.id; b = np * 4; break;
+    case RT_AOV_T: p = s->aov.t; b = np * 4; break;
+    case RT_AOV_NORMAL: p = s->aov.normal; b = np * 12; break;
+    case RT_AOV_DEPTH: p = s->aov.depth; b = np * 4; break;
+    case RT_AOV_ALBEDO: p = s->aov.albedo; b = np * 12; break;
+    case RT_AOV_MASK: p = s->aov.mask; b = np * 4; break;
+    default:
+        set_error("unknown pass");
+        return RT_ERR_INVALID;
+    }
+    if (!p) {
+        set_error("no AOV render has run on this scene");
+        return RT_ERR_INVALID;
+    }
+    *dev_ptr = p;
+    if (bytes) *bytes = b;
+    return RT_OK;
+}
+
+}  // extern "C"

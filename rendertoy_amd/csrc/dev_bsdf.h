@@ -1,0 +1,337 @@
+// dev_bsdf.h -- Disney principled BSDF, emitter sampling and RNG on the device.
+//
+// Restates PrincipledBSDF::{Eval,Sample} and its lobes (src/material.cpp:35-295)
+// and the PBR helpers (src/rtmath.cpp:169-353) in the reference's operation
+// order; built with -ffp-contract=off.  Transcendentals are the device's own
+// sinf/cosf/powf/logf; sqrt and division are correctly rounded as on the host.
+#pragma once
+#include "dev_trace.h"
+
+namespace rtamd {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kE = 2.71828182845904523536f;   // M_Ef32
+
+__device__ __forceinline__ float lerpf_(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ V3 lerpv(V3 x, V3 y, V3 a) { return add(mulv(x, sub(v3(1.0f), a)), mulv(y, a)); }
+__device__ __forceinline__ float luma(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+__device__ __forceinline__ float fmax0(float x) { return (0.0f < x) ? x : 0.0f; }
+__device__ __forceinline__ bool approx_zero(V3 a) {
+    const float eps = 1e-7f;   // Vector operator== against O (rtmath.h:296-308)
+    if (fabsf(a.x - 0.0f) > eps) return false;
+    if (fabsf(a.y - 0.0f) > eps) return false;
+    if (fabsf(a.z - 0.0f) > eps) return false;
+    return true;
+}
+
+struct Mat {
+    V3 base, emission;
+    float roughness, metallic, spec_trans, ior, subsurface, specular_tint, sheen, sheen_tint, clearcoat,
+        clearcoat_roughness, at_distance;
+    V3 extinction;
+};
+__device__ __forceinline__ Mat load_mat(const rt_material &m) {
+    Mat r;
+    r.base = ld3(m.base_color);
+    r.emission = ld3(m.emission);
+    r.roughness = m.roughness;
+    r.metallic = m.metallic;
+    r.spec_trans = m.spec_trans;
+    r.ior = m.ior;
+    r.subsurface = m.subsurface;
+    r.specular_tint = m.specular_tint;
+    r.sheen = m.sheen;
+    r.sheen_tint = m.sheen_tint;
+    r.clearcoat = m.clearcoat;
+    r.clearcoat_roughness = m.clearcoat_roughness;
+    r.at_distance = m.at_distance;
+    r.extinction = ld3(m.extinction);
+    return r;
+}
+
+__device__ __forceinline__ float GTR1(float NdotH, float a) {
+    if (a >= 1.0f) return (1.0f / kPi);
+    float a2 = a * a;
+    float t = 1.0f + (a2 - 1.0f) * NdotH * NdotH;
+    return (a2 - 1.0f) / (kPi * logf(a2) * t);
+}
+__device__ __forceinline__ V3 SampleGTR1(float rgh, float r1) {
+    float a = (0.001f < rgh) ? rgh : 0.001f;
+    float a2 = a * a;
+    float phi = r1 * 2.0f * kPi;
+    float cos_theta = sqrtf((1.0f - powf(a2, 1.0f - r1)) / (1.0f - a2));
+    float st = sqrtf(1.0f - (cos_theta * cos_theta));
+    float sin_theta = (st < 0.0f) ? 0.0f : ((1.0f < st) ? 1.0f : st);
+    return {sin_theta * cosf(phi), sin_theta * sinf(phi), cos_theta};
+}
+__device__ __forceinline__ float GTR2(float NdotH, float a) {
+    float a2 = a * a;
+    float t = 1.0f + (a2 - 1.0f) * NdotH * NdotH;
+    return a2 / (kPi * t * t);
+}
+__device__ __forceinline__ V3 SampleGGXVNDF(V3 V, float rgh, float r1, float r2) {
+    V3 Vh = normalized(V3{rgh * V.x, rgh * V.y, V.z});
+    float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    V3 T1 = lensq > 0.0f ? muls(V3{-Vh.y, Vh.x, 0.0f}, 1.0f / sqrtf(lensq)) : V3{1.0f, 0.0f, 0.0f};
+    V3 T2 = cross(Vh, T1);
+    float r = sqrtf(r1);
+    float phi = 2.0f * kPi * r2;
+    float t1 = r * cosf(phi);
+    float t2 = r * sinf(phi);
+    float s = 0.5f * (1.0f + Vh.z);
+    t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
+    V3 Nh = add(add(muls(T1, t1), muls(T2, t2)), muls(Vh, sqrtf(fmax0(1.0f - t1 * t1 - t2 * t2))));
+    return normalized(V3{rgh * Nh.x, rgh * Nh.y, fmax0(Nh.z)});
+}
+__device__ __forceinline__ float SmithG(float NdotV, float alpha_g) {
+    float a = alpha_g * alpha_g;
+    float b = NdotV * NdotV;
+    return (2.0f * NdotV) / (NdotV + sqrtf(a + b - a * b));
+}
+__device__ __forceinline__ float SchlickFresnel(float u) {
+    float m = 1.0f - u;
+    m = (m < 0.0f) ? 0.0f : ((1.0f < m) ? 1.0f : m);
+    float m2 = m * m;
+    return m2 * m2 * m;
+}
+__device__ __forceinline__ float DielectricFresnel(float cos_theta_i, float eta) {
+    float sin_theta_t_sq = eta * eta * (1.0f - cos_theta_i * cos_theta_i);
+    if (sin_theta_t_sq > 1.0f) return 1.0f;
+    float x = 1.0f - sin_theta_t_sq;
+    float cos_theta_t = sqrtf((x < 0.0f) ? 0.0f : x);
+    float rs = (eta * cos_theta_t - cos_theta_i) / (eta * cos_theta_t + cos_theta_i);
+    float rp = (eta * cos_theta_i - cos_theta_t) / (eta * cos_theta_i + cos_theta_t);
+    return 0.5f * (rs * rs + rp * rp);
+}
+__device__ __forceinline__ V3 CosineSampleHemisphere(float r1, float r2) {
+    float r = sqrtf(r1);
+    float phi = 2.0f * kPi * r2;
+    V3 d;
+    d.x = r * cosf(phi);
+    d.y = r * sinf(phi);
+    d.z = sqrtf(fmax0(1.0f - d.x * d.x - d.y * d.y));
+    return d;
+}
+__device__ __forceinline__ float PowerHeuristic(float a, float b) {
+    float t = a * a;
+    return t / (b * b + t);
+}
+__device__ __forceinline__ void Onb(V3 N, V3 &T, V3 &B) {
+    V3 up = fabsf(N.z) < 0.999f ? V3{0.0f, 0.0f, 1.0f} : V3{1.0f, 0.0f, 0.0f};
+    T = normalized(cross(up, N));
+    B = cross(N, T);
+}
+__device__ __forceinline__ V3 ToWorld(V3 X, V3 Y, V3 Z, V3 V) {
+    return add(add(muls(X, V.x), muls(Y, V.y)), muls(Z, V.z));
+}
+__device__ __forceinline__ V3 ToLocal(V3 X, V3 Y, V3 Z, V3 V) { return {dot(V, X), dot(V, Y), dot(V, Z)}; }
+__device__ __forceinline__ V3 Reflect(V3 I, V3 N) { return sub(I, muls(N, 2.0f * dot(I, N))); }
+__device__ __forceinline__ V3 Refract(V3 I, V3 N, float eta) {
+    float NdotI = dot(N, I);
+    float k = 1.0f - eta * eta * (1.0f - NdotI * NdotI);
+    if (k < 0.0f) return {0.0f, 0.0f, 0.0f};
+    return sub(muls(I, eta), muls(N, eta * NdotI + sqrtf(k)));
+}
+
+__device__ __forceinline__ float FresnelMix(const Mat &m, float eta, float VdotH) {
+    float metallic_fresnel = SchlickFresnel(VdotH);
+    float dielectric_fresnel = DielectricFresnel(VdotH, eta);
+    return lerpf_(dielectric_fresnel, metallic_fresnel, m.metallic);
+}
+__device__ __forceinline__ V3 EvalDiffuse(const Mat &m, V3 Csheen, V3 V, V3 L, V3 H, float &pdf) {
+    pdf = 0.0f;
+    if (L.z <= 0.0f) return v3(0.0f);
+    float LH = dot(L, H);
+    float FL = SchlickFresnel(L.z);
+    float FV = SchlickFresnel(V.z);
+    float FH = SchlickFresnel(LH);
+    float Fd90 = 0.5f + 2.0f * LH * LH * m.roughness;
+    float Fd = lerpf_(1.0f, Fd90, FL) * lerpf_(1.0f, Fd90, FV);
+    float Fss90 = LH * LH * m.roughness;
+    float Fss = lerpf_(1.0f, Fss90, FL) * lerpf_(1.0f, Fss90, FV);
+    float ss = 1.25f * (Fss * (1.0f / (L.z + V.z) - 0.5f) + 0.5f);
+    V3 Fsheen = muls(Csheen, FH * m.sheen);
+    pdf = L.z * (1.0f / kPi);
+    V3 r = add(muls(m.base, (1.0f / kPi) * lerpf_(Fd, ss, m.subsurface)), Fsheen);
+    return muls(muls(r, 1.0f - m.metallic), 1.0f - m.spec_trans);
+}
+__device__ __forceinline__ V3 EvalSpecReflection(const Mat &m, float eta, V3 specCol, V3 V, V3 L, V3 H,
+                                                 float &pdf) {
+    pdf = 0.0f;
+    if (L.z <= 0.0f) return v3(0.0f);
+    float VH = dot(V, H);
+    float FM = FresnelMix(m, eta, dot(L, H));
+    V3 F = lerpv(specCol, v3(1.0f), v3(FM));
+    float D = GTR2(H.z, m.roughness);
+    float G1 = SmithG(fabsf(V.z), m.roughness);
+    float G2 = G1 * SmithG(fabsf(L.z), m.roughness);
+    float jacobian = 1.0f / (4.0f * VH);
+    pdf = G1 * fmax0(VH) * D * jacobian / V.z;
+    return divs(muls(muls(F, D), G2), 4.0f * L.z * V.z);
+}
+__device__ __forceinline__ V3 EvalSpecRefraction(const Mat &m, float eta, V3 V, V3 L, V3 H, float &pdf) {
+    pdf = 0.0f;
+    if (L.z >= 0.0f) return v3(0.0f);
+    float VH = dot(V, H), LH = dot(L, H);
+    float F = DielectricFresnel(fabsf(VH), eta);
+    float D = GTR2(H.z, m.roughness);
+    float denom = LH + VH * eta;
+    denom *= denom;
+    float G1 = SmithG(fabsf(V.z), m.roughness);
+    float G2 = G1 * SmithG(fabsf(L.z), m.roughness);
+    float jacobian = fabsf(LH) / denom;
+    pdf = G1 * fmax0(VH) * D * jacobian / V.z;
+    V3 r = {powf(m.base.x, 0.5f), powf(m.base.y, 0.5f), powf(m.base.z, 0.5f)};
+    r = muls(r, (float)(1.0 - (double)m.metallic));
+    r = muls(r, m.spec_trans);
+    r = muls(r, (float)(1.0 - (double)F));
+    r = muls(r, D);
+    r = muls(r, G2);
+    r = muls(r, fabsf(VH));
+    r = muls(r, fabsf(LH));
+    r = muls(r, eta);
+    r = muls(r, eta);
+    return divs(r, denom * fabsf(L.z) * fabsf(V.z));
+}
+__device__ __forceinline__ V3 EvalClearcoat(const Mat &m, V3 V, V3 L, V3 H, float &pdf) {
+    pdf = 0.0f;
+    if (L.z <= 0.0f) return v3(0.0f);
+    float VH = dot(V, H);
+    float FH = DielectricFresnel(VH, 1.0f / 1.5f);
+    float F = lerpf_(0.04f, 1.0f, FH);
+    float D = GTR1(H.z, m.clearcoat_roughness);
+    float G = SmithG(L.z, 0.25f) * SmithG(V.z, 0.25f);
+    float jacobian = 1.0f / (4.0f * VH);
+    pdf = D * H.z * jacobian;
+    V3 r = muls(muls(muls(muls(v3(0.25f), m.clearcoat), F), D), G);
+    return divs(r, 4.0f * L.z * V.z);
+}
+__device__ __forceinline__ void GetSpecColor(const Mat &m, float eta, V3 &spec, V3 &sheen) {
+    float lum = luma(m.base);
+    V3 ctint = lum > 0.0f ? divs(m.base, lum) : v3(1.0f);
+    float F0 = (1.0f - eta) / (1.0f + eta);
+    spec = lerpv(muls(lerpv(v3(1.0f), ctint, v3(m.specular_tint)), F0 * F0), m.base, v3(m.metallic));
+    sheen = lerpv(v3(1.0f), ctint, v3(m.sheen_tint));
+}
+__device__ __forceinline__ void GetLobeProbabilities(const Mat &m, V3 spec, float approx_fresnel, float &dw,
+                                                     float &srw, float &stw, float &cw) {
+    dw = luma(m.base) * (1.0f - m.metallic) * (1.0f - m.spec_trans);
+    srw = luma(lerpv(spec, v3(1.0f), v3(approx_fresnel)));
+    stw = (1.0f - approx_fresnel) * (1.0f - m.metallic) * m.spec_trans * luma(m.base);
+    cw = m.clearcoat * (1.0f - m.metallic);
+    float total = dw + srw + stw + cw;
+    dw /= total;
+    srw /= total;
+    stw /= total;
+    cw /= total;
+}
+// PrincipledBSDF::Eval (src/material.cpp:143-204)
+__device__ __forceinline__ V3 BsdfEval(const Mat &m, float eta, V3 V, V3 N, V3 L, float &bsdf_pdf) {
+    bsdf_pdf = 0.0f;
+    V3 f = v3(0.0f);
+    V3 T, B;
+    Onb(N, T, B);
+    V = ToLocal(T, B, N, V);
+    L = ToLocal(T, B, N, L);
+    V3 H;
+    if (L.z > 0.0f) H = normalized(add(L, V));
+    else H = normalized(add(L, muls(V, eta)));
+    if (H.z < 0.0f) H = neg(H);
+    V3 spec, sheen;
+    GetSpecColor(m, eta, spec, sheen);
+    float dw, srw, stw, cw;
+    float fresnel = FresnelMix(m, eta, dot(V, H));
+    GetLobeProbabilities(m, spec, fresnel, dw, srw, stw, cw);
+    float pdf;
+    if (dw > 0.0f && L.z > 0.0f) {
+        f = add(f, EvalDiffuse(m, sheen, V, L, H, pdf));
+        bsdf_pdf += pdf * dw;
+    }
+    if (srw > 0.0f && L.z > 0.0f && V.z > 0.0f) {
+        f = add(f, EvalSpecReflection(m, eta, spec, V, L, H, pdf));
+        bsdf_pdf += pdf * srw;
+    }
+    if (stw > 0.0f && L.z < 0.0f) {
+        f = add(f, EvalSpecRefraction(m, eta, V, L, H, pdf));
+        bsdf_pdf += pdf * stw;
+    }
+    if (cw > 0.0f && L.z > 0.0f && V.z > 0.0f) {
+        f = add(f, EvalClearcoat(m, V, L, H, pdf));
+        bsdf_pdf += pdf * cw;
+    }
+    return muls(f, fabsf(L.z));
+}
+// PrincipledBSDF::Sample (src/material.cpp:206-295); r1, r2 are its two draws.
+__device__ __forceinline__ bool BsdfSample(const Mat &m, float eta, V3 N, V3 in_dir, float r1, float r2,
+                                           V3 &out_dir, V3 &color, float &pdf) {
+    pdf = 0.0f;
+    V3 f = v3(0.0f);
+    V3 T, B;
+    Onb(N, T, B);
+    V3 V = ToLocal(T, B, N, in_dir);
+    V3 spec, sheen;
+    GetSpecColor(m, eta, spec, sheen);
+    float dw, srw, stw, cw;
+    float approx = FresnelMix(m, eta, V.z);
+    GetLobeProbabilities(m, spec, approx, dw, srw, stw, cw);
+    float cdf0 = dw, cdf1 = cdf0 + srw, cdf2 = cdf1 + stw;
+    if (r1 < cdf0) {
+        r1 /= cdf0;
+        out_dir = CosineSampleHemisphere(r1, r2);
+        V3 H = normalized(add(out_dir, V));
+        f = EvalDiffuse(m, sheen, V, out_dir, H, pdf);
+        pdf *= dw;
+    } else if (r1 < cdf1) {
+        r1 = (r1 - cdf0) / (cdf1 - cdf0);
+        V3 H = SampleGGXVNDF(V, m.roughness, r1, r2);
+        if (H.z < 0.0f) H = neg(H);
+        out_dir = normalized(Reflect(neg(V), H));
+        f = EvalSpecReflection(m, eta, spec, V, out_dir, H, pdf);
+        pdf *= srw;
+    } else if (r1 < cdf2) {
+        r1 = (r1 - cdf1) / (cdf2 - cdf1);
+        V3 H = SampleGGXVNDF(V, m.roughness, r1, r2);
+        if (H.z < 0.0f) H = neg(H);
+        out_dir = normalized(Refract(neg(V), H, eta));
+        f = EvalSpecRefraction(m, eta, V, out_dir, H, pdf);
+        pdf *= stw;
+    } else {
+        r1 = (float)((double)(r1 - cdf2) / (1.0 - (double)cdf2));
+        V3 H = SampleGTR1(m.clearcoat_roughness, r1);
+        if (H.z < 0.0f) H = neg(H);
+        out_dir = normalized(Reflect(neg(V), H));
+        f = EvalClearcoat(m, V, out_dir, H, pdf);
+        pdf *= cw;
+    }
+    out_dir = ToWorld(T, B, N, out_dir);
+    color = muls(f, fabsf(dot(N, out_dir)));
+    return !approx_zero(out_dir);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    }
+    return c;
+}
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+__device__ __forceinline__ uint32_t pick(uint32_t x, uint32_t n) { return __umulhi(x, n); }
+
+// SurfacePoint::GetEmission<true> (include/RenderToy/surfacepoint.h:20-45): the
+// emission of the triangle's material and the solid-angle pdf |d^2/(cos*area)|.
+__device__ __forceinline__ float emission_pdf(V3 position, V3 to_pos, V3 out_dir, V3 normal, float area) {
+    V3 ray = sub(to_pos, position);
+    float distance2 = dot(ray, ray);
+    float cos_area = dot(out_dir, normal) * area;
+    return fabsf(distance2 / cos_area);
+}
+
+}  // namespace rtamd

@@ -1,0 +1,294 @@
+// dev_trace.h -- device math and BVH4 traversal shared by all kernels.
+//
+// Float arithmetic follows RenderToy's Vector3f operation order
+// (include/RenderToy/rtmath.h:126-150,320-329,399-418) and the library is
+// compiled with -ffp-contract=off, so ray generation, Moller-Trumbore and the
+// shading formulas round exactly as the reference's IEEE single precision
+// code does.  Box culling is the only place that uses its own arithmetic; it
+// is padded so it can only keep extra boxes, never drop a hit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rendertoy_amd.h"
+#include "rt_layout.h"
+
+namespace rtamd {
+
+struct V3 { float x, y, z; };
+__device__ __forceinline__ V3 v3(float a) { return {a, a, a}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 neg(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 mulv(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 muls(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 divs(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) {
+    float r = 0.0f;
+    r += a.x * b.x;
+    r += a.y * b.y;
+    r += a.z * b.z;
+    return r;
+}
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length2(V3 a) {
+    float r = 0.0f;
+    r += a.x * a.x;
+    r += a.y * a.y;
+    r += a.z * a.z;
+    return r;
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(length2(a)); }
+__device__ __forceinline__ V3 normalized(V3 a) {
+    float l = length(a);
+    return {a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ V3 ld3(const float *p) { return {p[0], p[1], p[2]}; }
+
+// Matrix4x4f * (x, y, z, 1): row dots (rtmath.h:724-733).
+__device__ __forceinline__ float row_dot(const float *m, float x, float y, float z) {
+    float r = 0.0f;
+    r += m[0] * x;
+    r += m[1] * y;
+    r += m[2] * z;
+    r += m[3] * 1.0f;
+    return r;
+}
+
+// Primary ray of pixel (x, y): src/renderer.cpp:253-259 + Geometry::O2WTransform
+// (src/object.cpp:42-64).  divW/divH are 1.0f/float(W), 1.0f/float(H).
+__device__ __forceinline__ void camera_ray(const rt_camera &c, float divW, float divH, uint32_t x, uint32_t y,
+                                           V3 &src, V3 &dir) {
+    float ndc_x = float(x) * divW;
+    float ndc_y = float(y) * divH;
+    float sx = 2.0f * c.right * ndc_x - c.right;
+    float sy = 2.0f * c.top * ndc_y - c.top;
+    src = {row_dot(c.o2w + 0, 0.0f, 0.0f, 0.0f), row_dot(c.o2w + 4, 0.0f, 0.0f, 0.0f),
+           row_dot(c.o2w + 8, 0.0f, 0.0f, 0.0f)};
+    V3 p = {row_dot(c.o2w + 0, sx, sy, -1.0f), row_dot(c.o2w + 4, sx, sy, -1.0f), row_dot(c.o2w + 8, sx, sy, -1.0f)};
+    dir = normalized(sub(p, src));
+}
+
+// Two-sided Moller-Trumbore exactly as Triangle::Intersect (src/object.cpp:107-167).
+__device__ __forceinline__ bool mt_intersect(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float &t, float &u, float &v) {
+    V3 pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    V3 tvec = sub(o, v0);
+    float alpha = dot(pvec, tvec);
+    if (det > 0.0f) {
+        if (alpha < 0.0f || alpha > det) return false;
+        V3 qvec = cross(tvec, e1);
+        float beta = dot(qvec, d);
+        if (beta < 0.0f || alpha + beta > det) return false;
+        float inv_det = 1.0f / det;
+        t = dot(e2, qvec) * inv_det;
+        if (t < 0.0f) return false;
+        u = alpha * inv_det;
+        v = beta * inv_det;
+        return true;
+    } else if (det < 0.0f) {
+        if (alpha > 0.0f || alpha < det) return false;
+        V3 qvec = cross(tvec, e1);
+        float beta = dot(qvec, d);
+        if (beta > 0.0f || alpha + beta < det) return false;
+        float inv_det = 1.0f / det;
+        t = dot(e2, qvec) * inv_det;
+        if (t < 0.0f) return false;
+        u = alpha * inv_det;
+        v = beta * inv_det;
+        return true;
+    }
+    return false;
+}
+
+struct DevScene {
+    const Node4 *nodes;
+    const float4 *tris;        // 3 per triangle, BVH leaf order
+    const float4 *geo;         // normal.xyz, area by triangle index
+    const uint32_t *info;      // material | smooth << 31
+    const float *vnorm;        // 9 per triangle, or null
+    const rt_material *mats;
+    const EmitterRec *emit;
+    uint32_t num_emit;
+    uint32_t num_tris;
+    float sky[3], ground[3];
+};
+
+struct Hit {
+    uint32_t id;
+    float t, u, v;
+};
+
+// Traversal stack: the top kLdsStack entries of every lane live in LDS
+// (entry-major, stride = threads per block), deeper entries spill to a global
+// region (entry-major, stride = threads in the grid).
+constexpr int kLdsStack = 16;
+
+struct Stack {
+    uint32_t *lref;
+    float *lt;
+    int lstride;
+    uint32_t *gref;
+    float *gt;
+    int64_t gstride;
+    int sp;
+    __device__ __forceinline__ void push(uint32_t r, float t) {
+        if (sp < kLdsStack) {
+            lref[sp * lstride] = r;
+            lt[sp * lstride] = t;
+        } else {
+            gref[(int64_t)(sp - kLdsStack) * gstride] = r;
+            gt[(int64_t)(sp - kLdsStack) * gstride] = t;
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(uint32_t &r, float &t) {
+        --sp;
+        if (sp < kLdsStack) {
+            r = lref[sp * lstride];
+            t = lt[sp * lstride];
+        } else {
+            r = gref[(int64_t)(sp - kLdsStack) * gstride];
+            t = gt[(int64_t)(sp - kLdsStack) * gstride];
+        }
+    }
+};
+
+__device__ __forceinline__ float safe_rcp(float d) {
+    const float tiny = 1e-20f;
+    float dd = fabsf(d) < tiny ? copysignf(tiny, d) : d;
+    return 1.0f / dd;
+}
+
+// Relative slack on box distances; rounding of (lo - o) * (1/d) is at most a
+// few ulp, far below this.
+constexpr float kSlackLo = 1.0f - 1.0f / 262144.0f;
+constexpr float kSlackHi = 1.0f + 1.0f / 262144.0f;
+
+#define RT_CSWAP(a, b)                                                      \
+    if (tn[b] < tn[a]) {                                                    \
+        float _t = tn[a]; tn[a] = tn[b]; tn[b] = _t;                        \
+        uint32_t _r = rf[a]; rf[a] = rf[b]; rf[b] = _r;                     \
+    }
+
+// Closest hit (ANY = false): minimum (t, triangle index) over triangles other
+// than `ex0` with t < 1e7 -- the result of BVH::Intersect's brute-force form
+// (src/bvh.cpp:268-286).  Any hit (ANY = true): returns true as soon as a
+// triangle other than ex0/ex1 is hit with t < tmax.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t ex0, uint32_t ex1, float tmax,
+                                      Stack &stk, Hit &hit, uint32_t &n_nodes, uint32_t &n_tris) {
+    const float ix = safe_rcp(d.x), iy = safe_rcp(d.y), iz = safe_rcp(d.z);
+    float best = ANY ? tmax : kFloatInfinity;
+    uint32_t best_id = kNone;
+    float bu = 0.0f, bv = 0.0f;
+    uint32_t cur = 0;
+    stk.sp = 0;
+    while (true) {
+        if (!(cur & kLeafBit)) {
+            const float4 *np = reinterpret_cast<const float4 *>(sc.nodes + cur);
+            float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+            if (COUNT) ++n_nodes;
+            float lim = best * kSlackHi;
+            float tn[4];
+            uint32_t rf[4] = {ch.x, ch.y, ch.z, ch.w};
+            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float t0x = (lxa[j] - o.x) * ix, t1x = (hxa[j] - o.x) * ix;
+                float t0y = (lya[j] - o.y) * iy, t1y = (hya[j] - o.y) * iy;
+                float t0z = (lza[j] - o.z) * iz, t1z = (hza[j] - o.z) * iz;
+                float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+                float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+                bool h = rf[j] != kEmptyRef && a * kSlackLo <= b * kSlackHi && a * kSlackLo <= lim;
+                tn[j] = h ? a : __builtin_inff();
+            }
+            RT_CSWAP(0, 1);
+            RT_CSWAP(2, 3);
+            RT_CSWAP(0, 2);
+            RT_CSWAP(1, 3);
+            RT_CSWAP(1, 2);
+            if (tn[0] == __builtin_inff()) goto pop;
+            if (tn[3] != __builtin_inff()) stk.push(rf[3], tn[3]);
+            if (tn[2] != __builtin_inff()) stk.push(rf[2], tn[2]);
+            if (tn[1] != __builtin_inff()) stk.push(rf[1], tn[1]);
+            cur = rf[0];
+            continue;
+        } else {
+            uint32_t start = (cur >> 3) & 0x0FFFFFFFu;
+            uint32_t cnt = (cur & 7u) + 1u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 *tp = sc.tris + 3 * (size_t)(start + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                uint32_t id = __float_as_uint(c.y);
+                if (COUNT) ++n_tris;
+                if (id == ex0 || (ANY && id == ex1)) continue;
+                float t, u, v;
+                if (mt_intersect(o, d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v)) {
+                    if (ANY) {
+                        if (t < tmax) return true;
+                    } else if (t < kFloatInfinity && (t < best || (t == best && id < best_id))) {
+                        best = t;
+                        best_id = id;
+                        bu = u;
+                        bv = v;
+                    }
+                }
+            }
+        }
+    pop:
+        {
+            bool found = false;
+            float lim = best * kSlackHi;
+            while (stk.sp > 0) {
+                uint32_t r;
+                float t;
+                stk.pop(r, t);
+                if (t * kSlackLo <= lim) {
+                    cur = r;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) break;
+        }
+    }
+    hit.id = best_id;
+    hit.t = best;
+    hit.u = bu;
+    hit.v = bv;
+    return false;
+}
+#undef RT_CSWAP
+
+// Triangle::NormalC (src/object.cpp:207-217).
+__device__ __forceinline__ V3 shading_normal(const DevScene &sc, uint32_t id, float u, float v) {
+    uint32_t inf = sc.info[id];
+    if (inf & 0x80000000u) {
+        const float *n = sc.vnorm + 9 * (size_t)id;
+        float w = 1.0f - u - v;
+        return normalized(add(add(muls(ld3(n), w), muls(ld3(n + 3), u)), muls(ld3(n + 6), v)));
+    }
+    float4 g = sc.geo[id];
+    return {g.x, g.y, g.z};
+}
+
+// XCD-aware work distribution: the tile range is cut into 8 contiguous bands,
+// band k first served to the workgroups that land on XCD k (round-robin
+// dispatch puts workgroup b on XCD b % 8); a drained band is then helped by
+// the others.  One atomic per wave per tile.
+struct TileQueue {
+    uint32_t *counters;   // 8 counters, each on its own 128-byte line (stride 32 words)
+    uint32_t ntiles;
+};
+
+__device__ __forceinline__ uint32_t band_begin(uint32_t ntiles, uint32_t b) {
+    return (uint32_t)(((uint64_t)ntiles * b) / 8u);
+}
+
+}  // namespace rtamd

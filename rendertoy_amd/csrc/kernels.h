@@ -1,0 +1,27 @@
+// kernels.h -- host-callable launchers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rendertoy_amd.h"
+
+namespace rtamd {
+
+struct DevScene;
+
+struct AovDev {
+    uint32_t *id;
+    float *t;
+    float *normal;
+    float *depth;
+    float *albedo;
+    float *mask;
+};
+
+hipError_t launch_reset(uint32_t *p, uint32_t n, hipStream_t s);
+int aov_blocks_per_cu(bool count);
+hipError_t launch_aov(const DevScene &sc, const rt_camera &cam, const AovDev &out, uint32_t passes,
+                      uint32_t *counters, uint32_t grid, uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride,
+                      unsigned long long *stat, bool count, hipStream_t s);
+
+}  // namespace rtamd

@@ -1,0 +1,321 @@
+// pt.hip -- wavefront path tracer kernels.
+//
+// Restates PathTracingRenderer::Render / Radiance / DirectLight
+// (src/renderer.cpp:236-384) as a wavefront over a batch of paths:
+//
+//   primary  -- one closest hit per pixel; every sample of a pixel shares the
+//               same pixel-corner ray (renderer.cpp:255-259), so it is traced once.
+//   shade d  -- emission with MIS, emitter sample + BSDF eval for the shadow
+//               ray, BSDF sample for the next ray (compacted into queue d+1).
+//   shadow d -- any-hit visibility of the queued shadow rays; visible rays add
+//               their contribution to the path's radiance.
+//   extend d -- closest hit of queue d.
+//   accum    -- buffer += L/spp in sample order per pixel (renderer.cpp:261).
+//
+// A path slot i of a batch is pixel i % npix, sample s0 + i / npix.  All
+// randomness is Philox4x32-10 on (pixel, sample, depth, k), so the image does
+// not depend on batch size, queue order or launch shape.
+#include "dev_bsdf.h"
+#include "pt_kernels.h"
+
+namespace rtamd {
+
+// Per-batch counter block: for depth d, 4 counters (queue size, extend fetch,
+// shadow queue size, shadow fetch), each on its own 128-byte line.
+__device__ __forceinline__ uint32_t *ctr_at(const PtDev &p, uint32_t batch, uint32_t d, uint32_t k) {
+    return p.ctr + (((size_t)batch * kMaxDepth + d) * 4u + k) * 32u;
+}
+
+// Wave-aggregated append: one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    const uint32_t total = (uint32_t)__popcll(mask);
+    const uint32_t leader = mask ? (uint32_t)(__ffsll((unsigned long long)mask) - 1) : 0u;
+    if (lane == leader && total) base = atomicAdd(counter, total);
+    base = __shfl(base, (int)leader);
+    const uint64_t below = mask & ((1ull << lane) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+__global__ __launch_bounds__(256) void pt_primary_kernel(DevScene sc, PtDev p, uint32_t tiles_x, uint32_t ntiles,
+                                                         uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
+    __shared__ uint32_t s_ref[kLdsStack * 256];
+    __shared__ float s_t[kLdsStack * 256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
+    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    const uint32_t xcd = blockIdx.x & 7u;
+    uint32_t nn = 0, nt = 0;
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint32_t band = (xcd + k) & 7u;
+        const uint32_t b0 = band_begin(ntiles, band), b1 = band_begin(ntiles, band + 1);
+        while (true) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(p.prim_ctr + band * 32u, 1u);
+            t = __shfl(t, 0);
+            if (t >= b1 - b0) break;
+            const uint32_t tile = b0 + t;
+            const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u);
+            const uint32_t y = (tile / tiles_x) * 8u + (lane >> 3);
+            if (x >= p.cam.width || y >= p.cam.height) continue;
+            V3 o, d;
+            camera_ray(p.cam, p.divW, p.divH, x, y, o, d);
+            Hit h;
+            trace<false, false>(sc, o, d, kNone, kNone, 0.0f, stk, h, nn, nt);
+            p.prim[(size_t)y * p.cam.width + x] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+        }
+    }
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t s0,
+                                                       uint32_t npaths, uint32_t depth) {
+    const uint32_t count = FIRST ? npaths : *ctr_at(p, batch, depth, 0);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const bool last = depth + 1 >= p.max_depth;
+    uint32_t *q_out = (depth & 1u) ? p.q0 : p.q1;
+    const uint32_t *q_in = (depth & 1u) ? p.q1 : p.q0;
+    uint32_t *next_ctr = last ? nullptr : ctr_at(p, batch, depth + 1, 0);
+    uint32_t *sh_ctr = ctr_at(p, batch, depth, 2);
+    // Iterate whole waves so the wave-level appends see every lane.
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t j = base + threadIdx.x;
+        const bool active = j < count;
+        bool push_next = false, push_shadow = false;
+        uint32_t slot = 0;
+        if (active) {
+            slot = FIRST ? j : q_in[j];
+            const uint32_t pixel = slot % p.npix;
+            const uint32_t sample = s0 + slot / p.npix;
+            V3 o, d;
+            float last_pdf;
+            uint32_t hid;
+            float ht, hu, hv;
+            V3 T, L, absn;
+            if (FIRST) {
+                camera_ray(p.cam, p.divW, p.divH, pixel % p.cam.width, pixel / p.cam.width, o, d);
+                last_pdf = 0.0f;
+                float4 h = p.prim[pixel];
+                ht = h.x; hu = h.y; hv = h.z; hid = __float_as_uint(h.w);
+                T = v3(1.0f);
+                L = v3(0.0f);
+                absn = v3(0.0f);
+            } else {
+                float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
+                float4 tt = p.thr[slot], ll = p.rad[slot], ab = p.absn[slot];
+                o = {ro.x, ro.y, ro.z};
+                last_pdf = ro.w;
+                d = {rd.x, rd.y, rd.z};
+                ht = h.x; hu = h.y; hv = h.z; hid = __float_as_uint(h.w);
+                T = {tt.x, tt.y, tt.z};
+                L = {ll.x, ll.y, ll.z};
+                absn = {ab.x, ab.y, ab.z};
+            }
+            if (hid == kNone) {
+                // World::GetDefaultEmission(-dir) (src/world.cpp:39-42)
+                V3 bg = (-d.z < 0.0f) ? ld3(sc.sky) : ld3(sc.ground);
+                L = add(L, mulv(T, bg));
+            } else {
+                const uint4 ra = philox(make_uint4(pixel, sample, depth, 0u), p.key0, p.key1);
+                const uint32_t minfo = sc.info[hid];
+                const Mat m = load_mat(sc.mats[minfo & 0x7FFFFFFFu]);
+                const V3 pos = add(o, muls(d, ht));
+                const V3 N = shading_normal(sc, hid, hu, hv);
+                V3 ff = N;
+                float eta;
+                if (dot(d, ff) > 0.0f) {
+                    ff = neg(ff);
+                    eta = m.ior;
+                    absn = v3(0.0f);
+                } else {
+                    eta = 1.0f / m.ior;
+                }
+                const float area = sc.geo[hid].w;
+                const float pdf_e = emission_pdf(pos, o, neg(d), N, area);
+                V3 e_term = depth == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
+                L = add(L, mulv(T, e_term));
+                const float len = length(sub(pos, o));
+                const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
+                // DirectLight (src/renderer.cpp:335-384)
+                if (sc.num_emit > 0) {
+                    const EmitterRec &E = sc.emit[pick(ra.x, sc.num_emit)];
+                    const float sqr1 = sqrtf(u01(ra.y));
+                    const float r2 = u01(ra.z);
+                    const float ea = 1.0f - sqr1, eb = (1.0f - r2) * sqr1;
+                    const V3 epos = add(add(muls(ld3(E.e1), ea), muls(ld3(E.e2), eb)), ld3(E.v0));
+                    const V3 dir_to = normalized(sub(epos, pos));
+                    float te = 0.0f, ue = 0.0f, ve = 0.0f, tmax = kFloatInfinity;
+                    if (E.tri != hid && mt_intersect(pos, dir_to, ld3(E.v0), ld3(E.e1), ld3(E.e2), te, ue, ve) &&
+                        te < kFloatInfinity) {
+                        tmax = te;
+                    } else {
+                        ue = 0.0f;
+                        ve = 0.0f;
+                    }
+                    const bool esmooth = (sc.info[E.tri] & 0x80000000u) != 0u;
+                    const V3 en = esmooth ? shading_normal(sc, E.tri, ue, ve) : ld3(E.normal);
+                    const float lightpdf = emission_pdf(epos, pos, neg(dir_to), en, E.area);
+                    float bsdfpdf;
+                    const V3 f = BsdfEval(m, eta, neg(d), ff, dir_to, bsdfpdf);
+                    const float w = PowerHeuristic(lightpdf, bsdfpdf);
+                    if (bsdfpdf > 0.0f) {
+                        const V3 em = ld3(sc.mats[sc.info[E.tri] & 0x7FFFFFFFu].emission);
+                        const V3 c = divs(muls(mulv(muls(f, w), em), (float)sc.num_emit), lightpdf);
+                        const V3 contrib = mulv(T, mulv(c, br));
+                        if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
+                            push_shadow = true;
+                            p.sh_d[slot] = make_float4(dir_to.x, dir_to.y, dir_to.z, tmax);
+                            p.sh_c[slot] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(E.tri));
+                        }
+                    }
+                }
+                V3 nd = v3(0.0f);
+                float bpdf = 0.0f;
+                if (!last) {
+                    const uint4 rb = philox(make_uint4(pixel, sample, depth, 1u), p.key0, p.key1);
+                    V3 color;
+                    if (BsdfSample(m, eta, ff, neg(d), u01(ra.w), u01(rb.x), nd, color, bpdf)) {
+                        absn = divs(neg(V3{logf(m.extinction.x), logf(m.extinction.y), logf(m.extinction.z)}),
+                                    m.at_distance);
+                        if (bpdf > 0.0f) {
+                            T = mulv(T, divs(mulv(br, color), bpdf));
+                            push_next = true;
+                        }
+                    }
+                }
+                if (push_next || push_shadow) {
+                    p.ray_o[slot] = make_float4(pos.x, pos.y, pos.z, bpdf);
+                    p.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(hid));
+                }
+                if (push_next) {
+                    p.thr[slot] = make_float4(T.x, T.y, T.z, 0.0f);
+                    p.absn[slot] = make_float4(absn.x, absn.y, absn.z, 0.0f);
+                }
+            }
+            p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+        }
+        const uint32_t qi = wave_append(next_ctr ? next_ctr : sh_ctr, push_next && next_ctr);
+        if (push_next && next_ctr) q_out[qi] = slot;
+        const uint32_t si = wave_append(sh_ctr, push_shadow);
+        if (push_shadow) p.sq[si] = slot;
+    }
+}
+
+template <bool SHADOW>
+__global__ __launch_bounds__(256) void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t depth,
+                                                       uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
+    __shared__ uint32_t s_ref[kLdsStack * 256];
+    __shared__ float s_t[kLdsStack * 256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
+    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    const uint32_t count = *ctr_at(p, batch, depth, SHADOW ? 2 : 0);
+    uint32_t *fetch = ctr_at(p, batch, depth, SHADOW ? 3 : 1);
+    const uint32_t *q = SHADOW ? p.sq : ((depth & 1u) ? p.q1 : p.q0);
+    uint32_t nn = 0, nt = 0;
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(fetch, 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        const uint32_t j = base + lane;
+        if (j >= count) continue;
+        const uint32_t slot = q[j];
+        const float4 ro = p.ray_o[slot];
+        const float4 rd = p.ray_d[slot];
+        const V3 o = {ro.x, ro.y, ro.z};
+        const uint32_t ex = __float_as_uint(rd.w);
+        Hit h;
+        if (SHADOW) {
+            const float4 sd = p.sh_d[slot];
+            const float4 c = p.sh_c[slot];
+            const bool occluded = trace<true, false>(sc, o, V3{sd.x, sd.y, sd.z}, ex, __float_as_uint(c.w), sd.w,
+                                                     stk, h, nn, nt);
+            if (!occluded) {
+                float4 l = p.rad[slot];
+                p.rad[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+            }
+        } else {
+            trace<false, false>(sc, o, V3{rd.x, rd.y, rd.z}, ex, kNone, 0.0f, stk, h, nn, nt);
+            p.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void pt_accum_kernel(PtDev p, uint32_t K) {
+    const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+    if (px >= p.npix) return;
+    const float inv = (float)p.spp;
+    V3 acc = {p.fb[3 * (size_t)px], p.fb[3 * (size_t)px + 1], p.fb[3 * (size_t)px + 2]};
+    for (uint32_t k = 0; k < K; ++k) {
+        const float4 l = p.rad[(size_t)k * p.npix + px];
+        acc = add(acc, divs(V3{l.x, l.y, l.z}, inv));
+    }
+    p.fb[3 * (size_t)px] = acc.x;
+    p.fb[3 * (size_t)px + 1] = acc.y;
+    p.fb[3 * (size_t)px + 2] = acc.z;
+}
+
+__global__ void pt_zero_kernel(float *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+__global__ void pt_zero_u32_kernel(uint32_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+int pt_trace_blocks_per_cu() {
+    int nb = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false>, 256, 0);
+    int nb2 = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true>, 256, 0);
+    int nb3 = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb3, pt_primary_kernel, 256, 0);
+    nb = nb < nb2 ? nb : nb2;
+    nb = nb < nb3 ? nb : nb3;
+    return nb > 0 ? nb : 1;
+}
+
+hipError_t pt_enqueue_frame_start(const PtDev &p, size_t ctr_words, hipStream_t s) {
+    size_t nf = (size_t)p.npix * 3;
+    hipLaunchKernelGGL(pt_zero_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, p.fb, nf);
+    hipLaunchKernelGGL(pt_zero_u32_kernel, dim3((unsigned)((ctr_words + 255) / 256)), dim3(256), 0, s, p.ctr,
+                       ctr_words);
+    hipLaunchKernelGGL(pt_zero_u32_kernel, dim3(1), dim3(256), 0, s, p.prim_ctr, (size_t)256);
+    return hipGetLastError();
+}
+
+hipError_t pt_enqueue_primary(const DevScene &sc, const PtDev &p, uint32_t grid, uint32_t *ovf_ref, float *ovf_t,
+                              int64_t ovf_stride, hipStream_t s) {
+    const uint32_t tiles_x = (p.cam.width + 7) / 8, tiles_y = (p.cam.height + 7) / 8;
+    hipLaunchKernelGGL(pt_primary_kernel, dim3(grid), dim3(256), 0, s, sc, p, tiles_x, tiles_x * tiles_y, ovf_ref,
+                       ovf_t, ovf_stride);
+    return hipGetLastError();
+}
+
+hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t s0, uint32_t K,
+                            uint32_t trace_grid, uint32_t shade_grid, uint32_t *ovf_ref, float *ovf_t,
+                            int64_t ovf_stride, hipStream_t s) {
+    const uint32_t npaths = K * p.npix;
+    for (uint32_t d = 0; d < p.max_depth; ++d) {
+        if (d == 0) {
+            hipLaunchKernelGGL(pt_shade_kernel<true>, dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0, npaths,
+                               0u);
+        } else {
+            hipLaunchKernelGGL(pt_trace_kernel<false>, dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d, ovf_ref,
+                               ovf_t, ovf_stride);
+            hipLaunchKernelGGL(pt_shade_kernel<false>, dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0, npaths,
+                               d);
+        }
+        if (sc.num_emit > 0)
+            hipLaunchKernelGGL(pt_trace_kernel<true>, dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d, ovf_ref,
+                               ovf_t, ovf_stride);
+    }
+    hipLaunchKernelGGL(pt_accum_kernel, dim3((p.npix + 255) / 256), dim3(256), 0, s, p, K);
+    return hipGetLastError();
+}
+
+}  // namespace rtamd

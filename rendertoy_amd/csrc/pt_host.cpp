@@ -1,0 +1,233 @@
+// pt_host.cpp -- wavefront path tracer, host side: batching, buffers, hipGraph.
+//
+// A frame of W*H pixels at spp samples is cut into batches of K samples of
+// every pixel (K*W*H paths in flight).  The whole frame -- clear, primary
+// hits, then per batch the bounce loop and the accumulation -- is captured
+// once into a hipGraph and replayed while the camera and parameters stay the
+// same.  Queue sizes are read by the kernels from device counters, so the
+// graph does not depend on data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pt.h"
+#include "pt_kernels.h"
+#include "scene.h"
+
+namespace rtamd {
+
+struct PtState {
+    std::vector<DevBuf> bufs;
+    size_t cap = 0;          // path slots allocated
+    uint32_t npix = 0;       // pixels of the frame buffers allocated
+    size_t ctr_words = 0;
+    PtDev dev{};
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::vector<uint8_t> key;
+    uint32_t batches = 0, K = 0;
+};
+
+namespace {
+
+void hipchk(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+void *alloc(PtState *st, size_t bytes) {
+    DevBuf b;
+    b.bytes = bytes ? bytes : 16;
+    hipchk(hipMalloc(&b.p, b.bytes), "hipMalloc");
+    st->bufs.push_back(b);
+    return b.p;
+}
+
+void free_all(PtState *st) {
+    for (auto &b : st->bufs) (void)hipFree(b.p);
+    st->bufs.clear();
+    st->cap = 0;
+    st->npix = 0;
+    st->ctr_words = 0;
+    if (st->exec) (void)hipGraphExecDestroy(st->exec);
+    if (st->graph) (void)hipGraphDestroy(st->graph);
+    st->exec = nullptr;
+    st->graph = nullptr;
+    st->key.clear();
+}
+
+}  // namespace
+
+void pt_destroy(PtState *p) {
+    if (!p) return;
+    free_all(p);
+    delete p;
+}
+
+int pt_device_output(rt_scene *s, void **dev_ptr, size_t *bytes) {
+    if (!s->pt || !s->pt->dev.fb) {
+        set_error("no path-traced frame on this scene");
+        return RT_ERR_INVALID;
+    }
+    *dev_ptr = s->pt->dev.fb;
+    if (bytes) *bytes = (size_t)s->pt->npix * 3 * sizeof(float);
+    return RT_OK;
+}
+
+}  // namespace rtamd
+
+using namespace rtamd;
+
+extern "C" int rt_render_pt(rt_scene *s, const rt_camera *cam, const rt_pt_params *prm, float *rgb_host) {
+    try {
+        if (!s || !cam || !prm) {
+            set_error("scene, camera or params is NULL");
+            return RT_ERR_INVALID;
+        }
+        if (cam->width == 0 || cam->height == 0 || (uint64_t)cam->width * cam->height >= (1ull << 31)) {
+            set_error("bad frame size");
+            return RT_ERR_INVALID;
+        }
+        if (prm->spp == 0 || prm->max_depth == 0 || prm->max_depth > kMaxDepth) {
+            set_error("spp must be >= 1 and max_depth in [1, 64]");
+            return RT_ERR_INVALID;
+        }
+        hipchk(hipSetDevice(s->device), "hipSetDevice");
+        if (!s->pt) s->pt = new PtState();
+        PtState *st = s->pt;
+        const uint32_t npix = cam->width * cam->height;
+        // batch size: about 4M paths in flight unless the caller says otherwise
+        uint64_t target = prm->batch_paths ? prm->batch_paths : (4u << 20);
+        uint32_t K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(prm->spp, target / npix));
+        uint32_t nb = (prm->spp + K - 1) / K;
+        if (nb > kMaxBatches) {
+            K = (prm->spp + kMaxBatches - 1) / kMaxBatches;
+            nb = (prm->spp + K - 1) / K;
+        }
+        const size_t cap = (size_t)K * npix;
+        if (cap >= (1ull << 32)) {
+            set_error("batch too large");
+            return RT_ERR_INVALID;
+        }
+        const size_t ctr_words = (size_t)nb * kMaxDepth * 4 * 32;
+        if (st->cap < cap || st->npix != npix || st->ctr_words < ctr_words) {
+            free_all(st);
+            PtDev &p = st->dev;
+            p.ray_o = (float4 *)alloc(st, cap * 16);
+            p.ray_d = (float4 *)alloc(st, cap * 16);
+            p.thr = (float4 *)alloc(st, cap * 16);
+            p.rad = (float4 *)alloc(st, cap * 16);
+            p.absn = (float4 *)alloc(st, cap * 16);
+            p.hit = (float4 *)alloc(st, cap * 16);
+            p.sh_d = (float4 *)alloc(st, cap * 16);
+            p.sh_c = (float4 *)alloc(st, cap * 16);
+            p.q0 = (uint32_t *)alloc(st, cap * 4);
+            p.q1 = (uint32_t *)alloc(st, cap * 4);
+            p.sq = (uint32_t *)alloc(st, cap * 4);
+            p.prim = (float4 *)alloc(st, (size_t)npix * 16);
+            p.fb = (float *)alloc(st, (size_t)npix * 12);
+            p.ctr = (uint32_t *)alloc(st, ctr_words * 4);
+            p.prim_ctr = (uint32_t *)alloc(st, 256 * 4);
+            st->cap = cap;
+            st->npix = npix;
+            st->ctr_words = ctr_words;
+        }
+        PtDev &p = st->dev;
+        p.cam = *cam;
+        p.divW = 1.0f / float(cam->width);
+        p.divH = 1.0f / float(cam->height);
+        p.npix = npix;
+        p.spp = prm->spp;
+        p.max_depth = prm->max_depth;
+        p.key0 = (uint32_t)prm->seed;
+        p.key1 = (uint32_t)(prm->seed >> 32);
+
+        const uint32_t trace_grid = persistent_grid(s, pt_trace_blocks_per_cu());
+        ensure_overflow(s, trace_grid * 256);
+        const int64_t ovf_stride = (int64_t)trace_grid * 256;
+        const uint32_t shade_grid = (uint32_t)std::min<uint64_t>((cap + 255) / 256, (uint64_t)s->cu_count * 8);
+
+        auto enqueue = [&](hipStream_t strm) {
+            hipchk(pt_enqueue_frame_start(p, ctr_words, strm), "frame start");
+            hipchk(pt_enqueue_primary(s->ds, p, trace_grid, s->d_ovf_ref, s->d_ovf_t, ovf_stride, strm), "primary");
+            for (uint32_t b = 0; b < nb; ++b) {
+                uint32_t s0 = b * K;
+                uint32_t Kb = std::min(K, prm->spp - s0);
+                hipchk(pt_enqueue_batch(s->ds, p, b, s0, Kb, trace_grid, shade_grid, s->d_ovf_ref, s->d_ovf_t,
+                                        ovf_stride, strm),
+                       "batch");
+            }
+        };
+
+        // graph key: everything the captured launches depend on
+        std::vector<uint8_t> key(sizeof(PtDev) + 64);
+        std::memcpy(key.data(), &p, sizeof(PtDev));
+        uint64_t extra[8] = {nb, K, trace_grid, shade_grid, (uint64_t)(uintptr_t)s->d_ovf_ref,
+                             (uint64_t)(uintptr_t)s->d_ovf_t, (uint64_t)ovf_stride, ctr_words};
+        std::memcpy(key.data() + sizeof(PtDev), extra, sizeof(extra));
+        const bool use_graph = !(prm->flags & RT_PT_NO_GRAPH);
+
+        hipchk(hipEventRecord(s->ev[0], s->stream), "hipEventRecord");
+        if (use_graph) {
+            if (!st->exec || st->key != key) {
+                if (st->exec) (void)hipGraphExecDestroy(st->exec);
+                if (st->graph) (void)hipGraphDestroy(st->graph);
+                st->exec = nullptr;
+                st->graph = nullptr;
+                hipchk(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                try {
+                    enqueue(s->stream);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(s->stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                hipchk(hipStreamEndCapture(s->stream, &st->graph), "hipStreamEndCapture");
+                hipchk(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                st->key = key;
+                hipchk(hipEventRecord(s->ev[0], s->stream), "hipEventRecord");
+            }
+            hipchk(hipGraphLaunch(st->exec, s->stream), "hipGraphLaunch");
+        } else {
+            enqueue(s->stream);
+        }
+        hipchk(hipEventRecord(s->ev[1], s->stream), "hipEventRecord");
+        hipchk(hipStreamSynchronize(s->stream), "hipStreamSynchronize");
+        float ms = 0.0f;
+        hipchk(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]), "hipEventElapsedTime");
+        st->batches = nb;
+        st->K = K;
+
+        // ray counts from the per-batch counters
+        std::vector<uint32_t> ctr(ctr_words);
+        hipchk(hipMemcpy(ctr.data(), p.ctr, ctr_words * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        uint64_t rays = npix, shadow = 0;
+        for (uint32_t b = 0; b < nb; ++b)
+            for (uint32_t d = 0; d < prm->max_depth; ++d) {
+                const uint32_t *c = ctr.data() + (((size_t)b * kMaxDepth + d) * 4) * 32;
+                if (d > 0) rays += c[0];
+                shadow += c[2 * 32];
+            }
+        rays += shadow;
+        rt_stats &sts = s->stats;
+        sts.total_ms = ms;
+        sts.kernel_ms = ms;
+        sts.rays = rays;
+        sts.shadow_rays = shadow;
+        sts.primary_rays = npix;
+        sts.samples = (uint64_t)npix * prm->spp;
+        sts.node_visits = sts.tri_tests = 0;
+        if (rgb_host) hipchk(hipMemcpy(rgb_host, p.fb, (size_t)npix * 12, hipMemcpyDeviceToHost), "hipMemcpy");
+        return RT_OK;
+    } catch (const std::bad_alloc &) {
+        set_error("host out of memory");
+        return RT_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return RT_ERR_HIP;
+    }
+}

@@ -1,0 +1,42 @@
+// pt_kernels.h -- path-tracer device state and launchers.
+//
+// Path state, structure of arrays in HBM, one 16-byte record per path slot:
+//   ray_o  ox oy oz last_bsdfpdf      ray_d  dx dy dz exclude-triangle
+//   thr    throughput                 rad    accumulated radiance
+//   absn   RayState::absorption        hit    t u v triangle
+//   sh_d   shadow dir, t bound        sh_c   shadow contribution, emitter
+// Queues of slot indices: q0/q1 (extension, ping-pong by depth), sq (shadow).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rendertoy_amd.h"
+
+namespace rtamd {
+
+struct DevScene;
+
+constexpr uint32_t kMaxDepth = 64;      // path vertices
+constexpr uint32_t kMaxBatches = 4096;
+
+struct PtDev {
+    rt_camera cam;
+    float divW, divH;
+    uint32_t npix, spp, max_depth;
+    uint32_t key0, key1;
+    float4 *ray_o, *ray_d, *thr, *rad, *absn, *hit, *sh_d, *sh_c, *prim;
+    uint32_t *q0, *q1, *sq;
+    uint32_t *ctr;        // kMaxDepth * 4 counters per batch, 32-word stride
+    uint32_t *prim_ctr;   // 8 band counters, 32-word stride
+    float *fb;            // npix * 3
+};
+
+int pt_trace_blocks_per_cu();
+hipError_t pt_enqueue_frame_start(const PtDev &p, size_t ctr_words, hipStream_t s);
+hipError_t pt_enqueue_primary(const DevScene &sc, const PtDev &p, uint32_t grid, uint32_t *ovf_ref, float *ovf_t,
+                              int64_t ovf_stride, hipStream_t s);
+hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t s0, uint32_t K,
+                            uint32_t trace_grid, uint32_t shade_grid, uint32_t *ovf_ref, float *ovf_t,
+                            int64_t ovf_stride, hipStream_t s);
+
+}  // namespace rtamd

@@ -1,0 +1,52 @@
+// scene.h -- the opaque rt_scene behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/rendertoy_amd.h"
+#include "dev_trace.h"
+#include "kernels.h"
+
+namespace rtamd {
+
+struct PtState;
+void set_error(const std::string &msg);
+
+// Device allocation owned by a scene; freed in ~rt_scene.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace rtamd
+
+struct rt_scene {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<rtamd::DevBuf> owned;   // scene geometry
+    rtamd::DevScene ds{};
+    uint32_t num_tris = 0;
+    uint32_t bvh_nodes = 0, bvh_depth = 0, max_stack = 0;
+    double build_ms = 0.0;
+    bool counting = false;
+    // work queue counters (8 x 128-byte lines) and statistics
+    uint32_t *d_counters = nullptr;
+    unsigned long long *d_stat = nullptr;
+    // traversal stack overflow region
+    uint32_t *d_ovf_ref = nullptr;
+    float *d_ovf_t = nullptr;
+    size_t ovf_bytes = 0;
+    // AOV outputs
+    rtamd::AovDev aov{};
+    uint32_t aov_w = 0, aov_h = 0;
+    std::vector<rtamd::DevBuf> aov_bufs;
+    // path tracer state
+    rtamd::PtState *pt = nullptr;
+    rt_stats stats{};
+
+    ~rt_scene();
+};

@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X path-tracing hot path; prints ONE JSON line.
+
+Workload (N=1): the 1,048,576-triangle primary-ray AOV raycast -- 64 jittered
+geodesic spheres of 16,384 triangles, 1920x1080, 1 sample per pixel, normal +
+albedo + depth passes plus hit ids and distances, all in one launch.  A step
+is one such frame with the scene resident in HBM and outputs left on the
+device.  The same line carries the plain path tracer on the 36-triangle
+Cornell box (256x256, 16 spp, depth 5) and on the 2.4M-triangle terrain
+(1920x1080, 64 spp, depth 8) under "path_tracer".
+
+roofline: dominant kernel = the AOV traversal kernel.  Algorithmic bytes per
+ray = 128 B per BVH4 node fetched + 48 B per triangle tested + 36 B of AOV
+output + 68 B of hit shading data (normal/area, material index, vertex
+normals, base colour), with node and triangle counts measured by the same
+kernel in counting mode on the same frame; achieved = bytes per launch /
+mean launch time from HIP events over the timed steps.
+
+cpu_baseline: RenderToy itself (oracle/_ref, compiled from the reference's
+sources) on the host cores: BVH::Intersect on the exact AlbedoRenderer rays
+of a bounded pixel sample; falls back to the oracle restatement when the
+reference build is absent.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Msamples/s at 1 GPU; equal-time RMSE vs 65k-spp oracle (cached vs plain)"
+HBM_PEAK_GBS = 8000.0
+B_NODE, B_TRI, B_OUT, B_HIT = 128, 48, 36, 68
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline_c1(sc, seconds, max_rays=1 << 15):
+    """Reference (or oracle) closest-hit on the AlbedoRenderer rays of a
+    stratified pixel sample, timed on all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import py_oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    W, H = sc.width, sc.height
+    top, right = sc.camera.screen_space(W, H)
+    orc = py_oracle.Oracle()
+    src, d = orc.camera_rays(sc.camera.o2w, top, right, W, H)
+    rng = np.random.default_rng(17)
+    order = rng.permutation(W * H)
+    kind = "reference"
+    try:
+        ref = py_oracle.RefLib()
+        world = ref.world(sc)
+        run = lambda idx: world.intersect(src[idx], d[idx])
+    except (FileNotFoundError, OSError):
+        kind = "port"
+        oscene = orc.scene(sc)
+        run = lambda idx: oscene.intersect(src[idx], d[idx], mode=1, threads=threads)
+    run(order[:64])                       # builds the octree outside the timed region
+    done, t0 = 0, time.perf_counter()
+    chunk = 256
+    while time.perf_counter() - t0 < seconds and done < max_rays:
+        run(order[done:done + chunk])
+        done += chunk
+    dt = time.perf_counter() - t0
+    return {"value": done / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": kind,
+            "sample": "%d of the 2,073,600 primary rays (random pixels) of the 1920x1080 frame, "
+                      "BVH::Intersect (octree) as AlbedoRenderer calls it, %.1f s" % (done, dt)}
+
+
+def cpu_baseline_c0(sc, spp):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import py_oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    try:
+        ref = py_oracle.RefLib()
+        w = ref.world(sc)
+        _, ms = w.render("pt", sc.width, sc.height, spp=spp)
+        kind = "reference"
+    except (FileNotFoundError, OSError):
+        o = py_oracle.Oracle().scene(sc)
+        t0 = time.perf_counter()
+        o.render_pt(spp=spp, rng="philox", mode=1, threads=threads)
+        ms = (time.perf_counter() - t0) * 1e3
+        kind = "port"
+    return {"value": sc.width * sc.height * spp / ms / 1e3, "unit": "Msamples/s", "cores": threads, "kind": kind,
+            "sample": "full %dx%d frame at %d spp, PathTracingRenderer::Render (OpenMP over samples)" % (
+                sc.width, sc.height, spp)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pt", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (3 times) to calibrate HBM counters")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch   # device-memory plumbing only: one known-size copy kernel
+        n = args.copy_calibration // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "launches": 3}))
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would render its own "
+              "rows or tiles with no exchange until a final gather); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from rendertoy_amd import capi, scenes
+
+    arch, cus = capi.device_info()
+    sc = scenes.spheres_c1()
+    s = capi.Scene(sc)
+    passes = capi.AOV_ID | capi.AOV_T | capi.AOV_NORMAL | capi.AOV_DEPTH | capi.AOV_ALBEDO
+    npix = sc.width * sc.height
+    for _ in range(args.warmup):
+        s.render_aov(passes=passes, download=False)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.render_aov(passes=passes, download=False)
+        kms.append(s.stats()["kernel_ms"])
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / args.steps
+    value = npix * args.steps / wall / 1e6
+    k_ms = float(np.mean(kms))
+    # algorithmic traffic of one launch from the counting variant of the same kernel
+    s.set_counting(True)
+    s.render_aov(passes=passes, download=False)
+    st = s.stats()
+    s.set_counting(False)
+    hits = s.render_aov(passes=capi.AOV_ID)["id"] != capi.NONE
+    n_node = st["node_visits"] / npix
+    n_tri = st["tri_tests"] / npix
+    bytes_launch = st["node_visits"] * B_NODE + st["tri_tests"] * B_TRI + npix * B_OUT + int(hits.sum()) * B_HIT
+    achieved = bytes_launch / (k_ms * 1e-3) / 1e9
+    # HBM traffic per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), see profiles/
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "aov_c1_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "C1 primary-ray AOV raycast: 1,048,576 triangles (64 geodesic spheres x 16,384), "
+                               "1920x1080 x 1 spp, normal+albedo+depth+id+t passes",
+                   "triangles": sc.num_triangles, "width": sc.width, "height": sc.height, "spp": 1,
+                   "scene_sha256_16": sc.fingerprint(), "device": arch, "compute_units": cus},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "aov_kernel", "kernel_ms": round(k_ms, 5),
+                     "per_ray": {"node_fetches": round(n_node, 3), "tri_tests": round(n_tri, 3),
+                                 "bytes": round(bytes_launch / npix, 1)}},
+    }
+    log("C1: %.1f Msamples/s, kernel %.4f ms, %.1f GB/s algorithmic" % (value, k_ms, achieved))
+    if not args.no_pt:
+        pt = {}
+        sc0 = scenes.cornell_c0()
+        s0 = capi.Scene(sc0)
+        for _ in range(3):
+            s0.render_pt(download=False)
+        t0 = time.perf_counter()
+        n0 = 20
+        rays = 0
+        for i in range(n0):
+            s0.render_pt(download=False)
+            rays += s0.stats()["rays"]
+        dt = time.perf_counter() - t0
+        pt["c0_cornell"] = {"workload": "36-tri Cornell box 256x256, 16 spp, depth 5",
+                            "msamples_per_s": round(sc0.width * sc0.height * sc0.spp * n0 / dt / 1e6, 2),
+                            "mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / n0, 3)}
+        if not args.no_cpu:
+            pt["c0_cornell"]["cpu_baseline"] = cpu_baseline_c0(sc0, sc0.spp)
+            pt["c0_cornell"]["speedup_vs_cpu"] = round(
+                pt["c0_cornell"]["msamples_per_s"] / pt["c0_cornell"]["cpu_baseline"]["value"], 1)
+        del s0
+        sc2 = scenes.terrain_c2()
+        s2 = capi.Scene(sc2)
+        s2.render_pt(download=False)
+        t0 = time.perf_counter()
+        n2 = 3
+        rays = 0
+        for i in range(n2):
+            s2.render_pt(download=False, seed=i)
+            rays += s2.stats()["rays"]
+        dt = time.perf_counter() - t0
+        pt["c2_terrain"] = {"workload": "2,424,832-tri Perlin terrain + 256 icospheres, 1920x1080, 64 spp, depth 8",
+                            "msamples_per_s": round(sc2.width * sc2.height * sc2.spp * n2 / dt / 1e6, 2),
+                            "mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / n2, 2)}
+        del s2
+        out["path_tracer"] = pt
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_c1(sc, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

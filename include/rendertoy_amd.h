@@ -119,6 +119,14 @@ typedef struct rt_aov_out {
  * names.  Blocks until done. */
 int rt_render_aov(rt_scene *scene, const rt_camera *cam, uint32_t passes, const rt_aov_out *out);
 
+/* Closest hit of a batch of rays: BVH::Intersect (include/RenderToy/bvh.h:74-86,
+ * src/bvh.cpp:266-341) for each ray (src[i], dir[i]) skipping triangle
+ * exclude[i] (0xFFFFFFFF or a NULL array = none).  out_hit[i] = {t, u, v, id}
+ * with id 0xFFFFFFFF and t = 1e7 on a miss.  Arrays are host memory
+ * (src/dir: n*3 floats, out_hit: n*4 words).  Blocks until done. */
+int rt_intersect(rt_scene *scene, uint32_t n, const float *src, const float *dir, const uint32_t *exclude,
+                 float *out_hit);
+
 /* Path tracer parameters.  max_depth counts path vertices; the reference
  * hardwires 5 (`depth > 4`, src/renderer.cpp:274). */
 typedef struct rt_pt_params {

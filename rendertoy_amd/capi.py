@@ -96,6 +96,7 @@ def lib():
     L.rt_scene_set_counting.argtypes = [vp, C.c_int]
     L.rt_scene_device_output.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_size_t)]
     L.rt_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
+    L.rt_intersect.argtypes = [vp, C.c_uint32, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -190,6 +191,17 @@ class Scene:
                              res["depth"].ctypes.data, res["albedo"].ctypes.data, res["mask"].ctypes.data)
         check(lib().rt_render_aov(self.h, C.byref(cam), passes, C.byref(out) if out is not None else None))
         return res
+
+    def intersect(self, src, d, exclude=None):
+        """Closest hits of explicit rays: (ids, t, u, v)."""
+        src = np.ascontiguousarray(src, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+        n = len(src)
+        out = np.zeros((n, 4), np.float32)
+        ex = None if exclude is None else np.ascontiguousarray(exclude, np.uint32)
+        check(lib().rt_intersect(self.h, n, src.ctypes.data, d.ctypes.data,
+                                 ex.ctypes.data if ex is not None else None, out.ctypes.data))
+        return out[:, 3].view(np.uint32).copy(), out[:, 0].copy(), out[:, 1].copy(), out[:, 2].copy()
 
     def render_pt(self, W=None, H=None, spp=None, max_depth=None, seed=0, batch_paths=0, flags=0, download=True):
         cam = self.camera(W, H)

@@ -82,6 +82,40 @@ __global__ __launch_bounds__(256) void aov_kernel(DevScene sc, rt_camera cam, Ao
     }
 }
 
+// BVH::Intersect over an explicit ray batch; waves fetch 64 rays at a time.
+__global__ __launch_bounds__(256) void ray_kernel(DevScene sc, const float *src, const float *dir,
+                                                  const uint32_t *exclude, float4 *out, uint32_t n,
+                                                  uint32_t *fetch, uint32_t *ovf_ref, float *ovf_t,
+                                                  int64_t ovf_stride) {
+    __shared__ uint32_t s_ref[kLdsStack * 256];
+    __shared__ float s_t[kLdsStack * 256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
+    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    uint32_t nn = 0, nt = 0;
+    while (true) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(fetch, 64u);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        if (i >= n) continue;
+        const V3 o = {src[3 * (size_t)i], src[3 * (size_t)i + 1], src[3 * (size_t)i + 2]};
+        const V3 d = {dir[3 * (size_t)i], dir[3 * (size_t)i + 1], dir[3 * (size_t)i + 2]};
+        Hit h;
+        trace<false, false>(sc, o, d, exclude ? exclude[i] : kNone, kNone, 0.0f, stk, h, nn, nt);
+        out[i] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+    }
+}
+
+hipError_t launch_rays(const DevScene &sc, const float *src, const float *dir, const uint32_t *exclude,
+                       float4 *out, uint32_t n, uint32_t *fetch, uint32_t grid, uint32_t *ovf_ref, float *ovf_t,
+                       int64_t ovf_stride, hipStream_t s) {
+    hipLaunchKernelGGL(ray_kernel, dim3(grid), dim3(256), 0, s, sc, src, dir, exclude, out, n, fetch, ovf_ref,
+                       ovf_t, ovf_stride);
+    return hipGetLastError();
+}
+
 __global__ void reset_u32_kernel(uint32_t *p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;

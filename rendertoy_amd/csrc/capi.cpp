@@ -145,6 +145,7 @@ rt_scene::~rt_scene() {
     if (d_stat) (void)hipFree(d_stat);
     if (d_ovf_ref) (void)hipFree(d_ovf_ref);
     if (d_ovf_t) (void)hipFree(d_ovf_t);
+    if (d_rays) (void)hipFree(d_rays);
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -384,6 +385,52 @@ int rt_render_aov(rt_scene *s, const rt_camera *cam, uint32_t passes, const rt_a
             if (passes & RT_AOV_ALBEDO) cp(out->albedo, s->aov.albedo, np * 12);
             if (passes & RT_AOV_MASK) cp(out->mask, s->aov.mask, np * 4);
         }
+        return (int)RT_OK;
+    });
+}
+
+int rt_intersect(rt_scene *s, uint32_t n, const float *src, const float *dir, const uint32_t *exclude,
+                 float *out_hit) {
+    return guarded([&]() {
+        if (!s) throw InvalidArg{"scene is NULL"};
+        if (n == 0) return (int)RT_OK;
+        if (!src || !dir || !out_hit) throw InvalidArg{"src, dir and out_hit are required"};
+        if (n >= (1u << 30)) throw InvalidArg{"too many rays in one call"};
+        RT_HIP(hipSetDevice(s->device));
+        auto a16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+        const size_t o_dir = a16((size_t)n * 12), o_ex = o_dir + a16((size_t)n * 12);
+        const size_t o_out = o_ex + a16((size_t)n * 4), need = o_out + (size_t)n * 16;
+        if (s->ray_bytes < need) {
+            if (s->d_rays) RT_HIP(hipFree(s->d_rays));
+            s->d_rays = nullptr;
+            s->ray_bytes = 0;
+            RT_HIP(hipMalloc(&s->d_rays, need));
+            s->ray_bytes = need;
+        }
+        char *base = (char *)s->d_rays;
+        float *d_src = (float *)base;
+        float *d_dir = (float *)(base + o_dir);
+        uint32_t *d_ex = (uint32_t *)(base + o_ex);
+        float4 *d_out = (float4 *)(base + o_out);
+        RT_HIP(hipMemcpyAsync(d_src, src, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
+        RT_HIP(hipMemcpyAsync(d_dir, dir, (size_t)n * 12, hipMemcpyHostToDevice, s->stream));
+        if (exclude) RT_HIP(hipMemcpyAsync(d_ex, exclude, (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
+        const uint32_t grid = persistent_grid(s, aov_blocks_per_cu(false));
+        ensure_ovf(s, grid * 256);
+        RT_HIP(launch_reset(s->d_counters, 8 * 32, s->stream));
+        RT_HIP(hipEventRecord(s->ev[0], s->stream));
+        RT_HIP(launch_rays(s->ds, d_src, d_dir, exclude ? d_ex : nullptr, d_out, n, s->d_counters, grid,
+                           s->d_ovf_ref, s->d_ovf_t, (int64_t)grid * 256, s->stream));
+        RT_HIP(hipEventRecord(s->ev[1], s->stream));
+        RT_HIP(hipMemcpyAsync(out_hit, d_out, (size_t)n * 16, hipMemcpyDeviceToHost, s->stream));
+        RT_HIP(hipStreamSynchronize(s->stream));
+        float ms = 0.0f;
+        RT_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+        s->stats.kernel_ms = s->stats.total_ms = ms;
+        s->stats.rays = n;
+        s->stats.samples = 0;
+        s->stats.shadow_rays = 0;
+        s->stats.primary_rays = 0;
         return (int)RT_OK;
     });
 }

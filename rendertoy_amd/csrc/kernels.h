@@ -24,4 +24,8 @@ hipError_t launch_aov(const DevScene &sc, const rt_camera &cam, const AovDev &ou
                       uint32_t *counters, uint32_t grid, uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride,
                       unsigned long long *stat, bool count, hipStream_t s);
 
+hipError_t launch_rays(const DevScene &sc, const float *src, const float *dir, const uint32_t *exclude,
+                       float4 *out, uint32_t n, uint32_t *fetch, uint32_t grid, uint32_t *ovf_ref, float *ovf_t,
+                       int64_t ovf_stride, hipStream_t s);
+
 }  // namespace rtamd

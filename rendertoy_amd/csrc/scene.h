@@ -40,6 +40,9 @@ struct rt_scene {
     uint32_t *d_ovf_ref = nullptr;
     float *d_ovf_t = nullptr;
     size_t ovf_bytes = 0;
+    // staging for rt_intersect
+    void *d_rays = nullptr;
+    size_t ray_bytes = 0;
     // AOV outputs
     rtamd::AovDev aov{};
     uint32_t aov_w = 0, aov_h = 0;

@@ -1,0 +1,58 @@
+"""Plain path tracer vs RenderToy's converged image (statistical parity).
+
+tests/golden/converged_c0.npz holds the C0 Cornell box (256x256, depth 5)
+rendered by PathTracingRenderer::Render itself in 8 single-threaded processes
+of 128 spp each (independent mt19937 offsets): the mean image of 1024 spp and
+the per-pixel standard deviation of a 128-spp estimate.  The GPU renders the
+same frame at 1024 spp with its Philox stream.  Both estimate the same
+expectation, so the difference must be noise:
+
+  * per-channel frame mean within 0.5 % of the reference's;
+  * RMSE(gpu - ref) within 1.25x of the RMSE predicted by the two images'
+    noise, sqrt(mean(2 * std^2 / 8));
+  * the GPU's own 1024-spp noise measured the same way (8 independent seeds
+    of 128 spp) within 15 % of the reference's (same estimator, same variance).
+
+Both renderers can produce a NaN sample: a shading point exactly in the plane
+of the sampled light triangle makes GetEmission's pdf |d^2/(cos*area)| infinite
+and PowerHeuristic(inf, pdf) = inf/inf (src/renderer.cpp:358-378).  The
+reference's own image has such a pixel; pixels non-finite in either image are
+excluded and their number is bounded (< 0.01 % of the frame).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "converged_c0.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(GOLD), reason="converged fixture not generated")
+def test_c0_matches_reference_statistically():
+    from rendertoy_amd import capi, scenes
+    g = np.load(GOLD, allow_pickle=False)
+    ref_mean, ref_std = g["mean"].astype(np.float64), g["std"].astype(np.float64)
+    procs, spp_each = int(g["procs"]), int(g["spp_each"])
+    sc = scenes.cornell_c0()
+    s = capi.Scene(sc)
+    runs = np.stack([s.render_pt(spp=spp_each, max_depth=5, seed=1000 + k).astype(np.float64)
+                     for k in range(procs)])
+    finite = np.isfinite(ref_mean).all(1) & np.isfinite(runs).all((0, 2))
+    print("non-finite pixels: ref", int((~np.isfinite(ref_mean).all(1)).sum()), "gpu",
+          int((~np.isfinite(runs).all((0, 2))).sum()), flush=True)
+    assert (~finite).sum() <= 1e-4 * len(finite)
+    runs, ref_mean, ref_std = runs[:, finite], ref_mean[finite], ref_std[finite]
+    gpu_mean = runs.mean(0)
+    gpu_std = runs.std(0, ddof=1)
+    m_ref, m_gpu = ref_mean.mean(0), gpu_mean.mean(0)
+    print("frame means ref", m_ref, "gpu", m_gpu, flush=True)
+    np.testing.assert_allclose(m_gpu, m_ref, rtol=5e-3)
+    rmse = np.sqrt(np.mean((gpu_mean - ref_mean) ** 2))
+    predicted = np.sqrt(np.mean((ref_std ** 2 + gpu_std ** 2) / procs))
+    print("rmse", rmse, "predicted from noise", predicted, flush=True)
+    assert rmse <= 1.25 * predicted
+    noise_ref = np.sqrt(np.mean(ref_std ** 2))
+    noise_gpu = np.sqrt(np.mean(gpu_std ** 2))
+    print("128-spp noise ref", noise_ref, "gpu", noise_gpu, flush=True)
+    assert abs(noise_gpu / noise_ref - 1.0) < 0.15

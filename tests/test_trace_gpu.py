@@ -1,0 +1,75 @@
+"""rt_intersect (BVH::Intersect over explicit rays) on the GPU.
+
+Pinned against closed forms on a dyadic triangle and against the oracle's
+brute-force closest hit (the reference's DISABLE_BVH form, bvh.cpp:268-286)
+on random rays, including rays with exactly zero direction components, rays
+starting inside the geometry and rays with an excluded triangle."""
+import numpy as np
+import pytest
+
+from helpers import bits, scene
+from test_oracle import NONE, one_triangle_scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def orc():
+    from py_oracle import Oracle
+    return Oracle()
+
+
+def test_moller_trumbore_closed_forms_gpu():
+    from rendertoy_amd import capi
+    s = capi.Scene(one_triangle_scene((0, 0, 0), (1, 0, 0), (0, 1, 0)))
+    cases = [((0.25, 0.25, 1.0), (0, 0, -1), (1.0, 0.25, 0.25)),
+             ((0.25, 0.5, -2.0), (0, 0, 1), (2.0, 0.25, 0.5)),
+             ((0.5, 0.5, 1.0), (0, 0, -1), (1.0, 0.5, 0.5)),
+             ((0.0, 0.0, 0.5), (0, 0, -1), (0.5, 0.0, 0.0)),
+             ((1.0, 0.0, 4.0), (0, 0, -1), (4.0, 1.0, 0.0)),
+             ((0.75, 0.5, 1.0), (0, 0, -1), None),
+             ((0.25, 0.25, 1.0), (0, 0, 1), None),
+             ((0.25, 0.25, 1.0), (1, 0, 0), None),
+             ((0.25, 0.25, 0.0), (0, 0, -1), (0.0, 0.25, 0.25)),
+             ((0.25, 0.25, 2e7), (0, 0, -1), None)]           # t >= 1e7 ignored
+    src = np.array([c[0] for c in cases], np.float32)
+    d = np.array([c[1] for c in cases], np.float32)
+    ids, t, u, v = s.intersect(src, d)
+    for i, c in enumerate(cases):
+        if c[2] is None:
+            assert ids[i] == NONE, i
+        else:
+            assert ids[i] == 0 and (t[i], u[i], v[i]) == tuple(np.float32(x) for x in c[2]), (i, t[i], u[i], v[i])
+    ids, *_ = s.intersect(src[:1], d[:1], exclude=np.array([0], np.uint32))
+    assert ids[0] == NONE
+
+
+@pytest.mark.parametrize("name", ["cornell", "zoo", "spheres_small", "spheres_flat"])
+def test_random_rays_match_brute_force(orc, name):
+    from rendertoy_amd import capi
+    sc = scene(name)
+    rng = np.random.default_rng(21)
+    n = 20000
+    lo, hi = sc.verts.reshape(-1, 3).min(0), sc.verts.reshape(-1, 3).max(0)
+    ext = hi - lo
+    src = rng.uniform(lo - 0.3 * ext, hi + 0.3 * ext, (n, 3)).astype(np.float32)
+    cen = sc.verts.mean(1)[rng.integers(0, sc.num_triangles, n)]
+    tgt = cen + rng.normal(0.0, 0.02 * ext.max(), (n, 3))
+    d = tgt - src
+    i = np.arange(n)
+    d[i % 3 == 0, 0] = 0.0          # exactly zero direction components (never all three)
+    d[i % 3 == 1, 1] = 0.0
+    d[(i % 3 == 2) & (i % 5 == 0), 2] = 0.0
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    ex = np.where(rng.uniform(size=n) < 0.3, rng.integers(0, sc.num_triangles, n), NONE).astype(np.uint32)
+    g = capi.Scene(sc).intersect(src, d, exclude=ex)
+    o = orc.scene(sc).intersect(src, d, exclude=ex, mode=0, threads=16)
+    bad = np.nonzero(g[0] != o[0])[0]
+    for i in bad[:8]:
+        print("mismatch", i, "src", src[i].tolist(), "dir", d[i].tolist(), "ex", ex[i], "gpu", g[0][i], g[1][i],
+              "oracle", o[0][i], o[1][i], flush=True)
+    assert len(bad) == 0
+    hit = o[0] != NONE
+    assert hit.mean() > 0.2
+    for k in (1, 2, 3):
+        assert np.array_equal(bits(g[k][hit]), bits(o[k][hit]))

@@ -15,6 +15,10 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "librendertoy_amd.so")
+API_LIB = os.path.join(OUT_DIR, "librendertoy.so")
+ROOT = os.path.dirname(HERE)
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-I" + os.path.join(ROOT, "include")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -46,9 +50,38 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(LIB + ".tmp", LIB)
+    # the RenderToy C++ API on top of the C ABI, and the OBJ render harness
+    _run([CXX, *HOST_FLAGS, "-fPIC", "-shared", os.path.join(HERE, "host", "rendertoy_api.cpp"), "-L" + OUT_DIR,
+          "-lrendertoy_amd", "-Wl,-rpath,$ORIGIN", "-o", API_LIB])
+    _run([CXX, *HOST_FLAGS, os.path.join(HERE, "demo", "render_obj.cpp"), "-L" + OUT_DIR, "-lrendertoy",
+          "-lrendertoy_amd", "-Wl,-rpath,$ORIGIN", "-o", os.path.join(OUT_DIR, "render_obj")])
     if verbose:
-        print("built", LIB)
+        print("built", LIB, API_LIB)
     return LIB
+
+
+def build_reference_demos(reference: str, verbose: bool = True) -> list:
+    """RenderToy's own demo programs (demo/CornellBox, demo/SmoothShading), compiled
+    unchanged against include/RenderToy/ and linked to this library.  Needs the
+    RenderToy sources; the binaries land next to the library."""
+    built = []
+    for d in ("CornellBox", "SmoothShading"):
+        src = os.path.join(reference, "demo", d, "main.cpp")
+        if not os.path.exists(src):
+            continue
+        out = os.path.join(OUT_DIR, "demo_" + d)
+        _run([CXX, *HOST_FLAGS, src, "-L" + OUT_DIR, "-lrendertoy", "-lrendertoy_amd", "-Wl,-rpath,$ORIGIN",
+              "-o", out])
+        built.append(out)
+    if verbose and built:
+        print("built", *built)
+    return built
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
 
 
 if __name__ == "__main__":

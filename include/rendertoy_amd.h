@@ -148,6 +148,68 @@ enum rt_pt_flags {
  * it is copied there as float[width*height][3].  Blocks until done. */
 int rt_render_pt(rt_scene *scene, const rt_camera *cam, const rt_pt_params *params, float *rgb_host);
 
+/* ---------------------------------------------------------------------------
+ * Radiance cache (no counterpart in RenderToy): a multiresolution hash-grid
+ * encoding and a 64-wide, 5-hidden-layer ReLU MLP (fp16 MFMA, fp32 accumulate,
+ * Adam), trained online on radiance samples the path tracer produces.
+ * Rendering paths end at vertex `first_vertex`: they keep that vertex's
+ * emission term and take the cache's prediction for everything after it.
+ * ------------------------------------------------------------------------- */
+typedef struct rt_cache rt_cache;
+
+typedef struct rt_cache_params {
+    uint32_t log2_table;        /* entries per level = 2^log2_table (19) */
+    uint32_t base_resolution;   /* coarsest grid resolution (16) */
+    uint32_t max_resolution;    /* finest grid resolution (2048) */
+    uint32_t train_batch;       /* samples per Adam step (65536) */
+    uint32_t train_paths;       /* training paths traced per rt_cache_train call */
+    uint32_t first_vertex;      /* vertex where rendering paths end in the cache (>= 1) */
+    float lr_mlp, lr_grid;      /* Adam learning rates */
+    float target_clamp;         /* training targets are clamped to this radiance */
+    uint64_t seed;              /* initialisation and training-path stream */
+} rt_cache_params;
+
+/* A cache over the scene's bounding box; weights initialised from seed.
+ * Destroying the scene first detaches its caches: every later call on such a
+ * cache fails with RT_ERR_INVALID except rt_cache_destroy and rt_cache_layout. */
+int rt_cache_create(rt_scene *scene, const rt_cache_params *params, rt_cache **out);
+int rt_cache_destroy(rt_cache *cache);
+
+/* Traces train_paths paths of up to max_depth vertices from random pixels of
+ * `cam` with the plain estimator and runs `steps` Adam steps on up to
+ * train_batch of their vertices (>= first_vertex).  *loss_out (may be NULL)
+ * receives the mean relative L2 loss of the last step. */
+int rt_cache_train(rt_cache *cache, const rt_camera *cam, uint32_t max_depth, uint32_t steps, float *loss_out);
+
+/* As rt_cache_train, on caller-given samples: n records of 20 floats,
+ * position 3, outgoing direction 3, facing normal 3, roughness, albedo 3,
+ * bounces left after the vertex (max_depth - 1 - vertex index), 0, 0,
+ * target radiance 3, 0. */
+int rt_cache_train_samples(rt_cache *cache, uint32_t n, const float *samples, uint32_t steps, float *loss_out);
+
+/* Predictions for n queries (records as above; the target is not read) into
+ * out (n x 4 floats, rgb + 0). */
+int rt_cache_eval(rt_cache *cache, uint32_t n, const float *queries, float *out);
+
+/* Loss and gradients of one batch without an update: grad_mlp (6 matrices,
+ * 5x64x64 + 16x64 floats, row = output) and, if not NULL, grad_grid
+ * (levels x 2^log2_table x 2 floats). */
+int rt_cache_grad(rt_cache *cache, uint32_t n, const float *samples, float *loss, float *grad_mlp,
+                  float *grad_grid);
+
+/* fp32 parameters: mlp as in rt_cache_grad; grid may be NULL. */
+int rt_cache_get_params(rt_cache *cache, float *mlp, float *grid);
+int rt_cache_set_params(rt_cache *cache, const float *mlp, const float *grid);
+
+/* Encoding layout: per level resolution and dense flag, and the scene box
+ * mapping x -> (x - bmin) * binv. */
+int rt_cache_layout(const rt_cache *cache, float *res, uint32_t *dense, float *bmin, float *binv);
+
+/* The path tracer with paths ending in the cache at params->first_vertex;
+ * as rt_render_pt otherwise. */
+int rt_render_pt_cached(rt_scene *scene, rt_cache *cache, const rt_camera *cam, const rt_pt_params *params,
+                        float *rgb_host);
+
 /* Counters and timings of the last render call on this scene. */
 typedef struct rt_stats {
     double total_ms;              /* device time of the whole call (HIP events) */

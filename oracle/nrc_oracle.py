@@ -1,0 +1,195 @@
+"""numpy restatement of the radiance cache (encoding, MLP, loss, gradients).
+
+TEST INFRASTRUCTURE ONLY: the checker for rendertoy_amd/csrc/nrc.hip.  It has
+no counterpart in RenderToy.  The device casts to fp16 at fixed points --
+encoded inputs, hidden activations, weights, the loss gradient and the
+back-propagated gradients fed to the matrix cores -- and accumulates in fp32;
+`forward`/`backward` with fp16=True apply the same casts, fp16=False runs the
+same mathematics in float64 (used for the finite-difference pin of
+`backward` itself).
+
+Records (20 floats): position 3, outgoing direction 3, facing normal 3,
+roughness, albedo 3, bounces left, 0, path slot, target 3, 0.
+Inputs x (64) = 16 grid levels x 2 features, 16 real spherical harmonics of
+the outgoing direction, normal*0.5+0.5 (3), roughness, albedo (3), bounces
+left / 16, zeros.  W0..W4 are 64x64, W5 is 16x64 (rows 3..15 zero); no biases;
+hidden ReLU; linear output.  Loss per sample and channel: (p - y)^2 / (p^2 +
+0.01) with p^2 + 0.01 held constant, averaged over 3n.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+LEVELS = 16
+N_PARAMS = 5 * 4096 + 16 * 64
+RECORD, BOUNCES, TARGET = 20, 13, 16
+
+
+def sh16(d):
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    f = np.float32
+    xx, yy, zz = x * x, y * y, z * z
+    return np.stack([
+        np.full_like(x, f(0.28209479177387814)),
+        f(-0.48860251190291987) * y,
+        f(0.48860251190291987) * z,
+        f(-0.48860251190291987) * x,
+        f(1.0925484305920792) * x * y,
+        f(-1.0925484305920792) * y * z,
+        f(0.94617469575755997) * zz - f(0.31539156525251999),
+        f(-1.0925484305920792) * x * z,
+        f(0.54627421529603959) * (xx - yy),
+        f(0.59004358992664352) * y * (f(-3.0) * xx + yy),
+        f(2.8906114426405538) * x * y * z,
+        f(0.45704579946446572) * y * (f(1.0) - f(5.0) * zz),
+        f(0.3731763325901154) * z * (f(5.0) * zz - f(3.0)),
+        f(0.45704579946446572) * x * (f(1.0) - f(5.0) * zz),
+        f(1.4453057213202769) * z * (xx - yy),
+        f(0.59004358992664352) * x * (-xx + f(3.0) * yy),
+    ], 1).astype(np.float32)
+
+
+class Layout:
+    @staticmethod
+    def make(log2_table, base_resolution, max_resolution, bbox):
+        """Level resolutions floor(base * b^l), b = (max/base)^(1/15); a level is
+        indexed densely when its (R+1)^3 vertices fit the table; the scene box
+        grows by 0.1% of its extent on each side."""
+        T = 1 << log2_table
+        b = np.exp((np.log(float(max_resolution)) - np.log(float(base_resolution))) / (LEVELS - 1))
+        res = np.floor(base_resolution * b ** np.arange(LEVELS, dtype=np.float64) + 1e-9)
+        dense = ((res + 1) ** 3 <= T).astype(np.uint32)
+        lo = np.asarray(bbox[:3], np.float32)
+        hi = np.asarray(bbox[3:], np.float32)
+        ext = np.maximum(hi - lo, np.float32(1e-6)).astype(np.float32)
+        bmin = (lo - np.float32(1e-3) * ext).astype(np.float32)
+        binv = (np.float32(1.0) / (ext * np.float32(1.002))).astype(np.float32)
+        return Layout(res.astype(np.float32), dense, bmin, binv, log2_table)
+
+    def __init__(self, res, dense, bmin, binv, log2_table):
+        self.res = np.asarray(res, np.float32)
+        self.dense = np.asarray(dense, np.uint32)
+        self.bmin = np.asarray(bmin, np.float32)
+        self.binv = np.asarray(binv, np.float32)
+        self.T = 1 << log2_table
+
+    def unit(self, pos):
+        return np.clip((pos.astype(np.float32) - self.bmin) * self.binv, np.float32(0), np.float32(1))
+
+    def corners(self, p, l):
+        """Table indices (n, 8) and trilinear weights (n, 8) of level l."""
+        R = self.res[l]
+        x = p * R
+        fl = np.minimum(np.floor(x), R - np.float32(1))
+        i0 = fl.astype(np.uint32)
+        fr = (x - fl).astype(np.float32)
+        idx = np.zeros((len(p), 8), np.uint32)
+        w = np.zeros((len(p), 8), np.float32)
+        side = np.uint32(self.res[l] + 1)
+        for k in range(8):
+            dx, dy, dz = k & 1, (k >> 1) & 1, (k >> 2) & 1
+            cx, cy, cz = i0[:, 0] + np.uint32(dx), i0[:, 1] + np.uint32(dy), i0[:, 2] + np.uint32(dz)
+            with np.errstate(over="ignore"):
+                if self.dense[l]:
+                    h = cx + side * (cy + side * cz)
+                else:
+                    h = cx ^ (cy * np.uint32(2654435761)) ^ (cz * np.uint32(805459861))
+            idx[:, k] = h & np.uint32(self.T - 1)
+            wx = fr[:, 0] if dx else np.float32(1) - fr[:, 0]
+            wy = fr[:, 1] if dy else np.float32(1) - fr[:, 1]
+            wz = fr[:, 2] if dz else np.float32(1) - fr[:, 2]
+            w[:, k] = (wx * wy * wz).astype(np.float32)
+        return idx, w
+
+
+def encode(lay: Layout, grid, q, fp16=True):
+    """(n, 64) encoded inputs.  grid: (LEVELS, T, 2) float32 masters."""
+    table = grid.astype(np.float16).astype(np.float32) if fp16 else grid.astype(np.float64)
+    p = lay.unit(q[:, 0:3])
+    n = len(q)
+    x = np.zeros((n, 64), np.float32 if fp16 else np.float64)
+    for l in range(LEVELS):
+        idx, w = lay.corners(p, l)
+        a = np.zeros(n, x.dtype)
+        b = np.zeros(n, x.dtype)
+        for k in range(8):
+            a = (a + w[:, k] * table[l, idx[:, k], 0]).astype(x.dtype)
+            b = (b + w[:, k] * table[l, idx[:, k], 1]).astype(x.dtype)
+        x[:, 2 * l] = a
+        x[:, 2 * l + 1] = b
+    x[:, 32:48] = sh16(q[:, 3:6].astype(np.float32))
+    x[:, 48:51] = q[:, 6:9] * np.float32(0.5) + np.float32(0.5)
+    x[:, 51] = q[:, 9]
+    x[:, 52:55] = q[:, 10:13]
+    x[:, 55] = q[:, BOUNCES] * np.float32(1.0 / 16.0)
+    if fp16:
+        x = x.astype(np.float16).astype(np.float32)
+    return x
+
+
+def split(mlp):
+    W = [mlp[j * 4096:(j + 1) * 4096].reshape(64, 64) for j in range(5)]
+    W.append(mlp[5 * 4096:].reshape(16, 64))
+    return W
+
+
+def _h(a, fp16):
+    return a.astype(np.float16).astype(np.float32) if fp16 else a
+
+
+def forward(lay, mlp, grid, q, fp16=True):
+    W = [_h(w, fp16) if fp16 else w.astype(np.float64) for w in split(mlp)]
+    x = encode(lay, grid, q, fp16)
+    acts = [x]
+    h = x
+    for j in range(5):
+        z = h @ W[j].T
+        h = _h(np.maximum(z, 0), fp16)
+        acts.append(h)
+    out = h @ W[5].T
+    return out[:, :3], acts, W
+
+
+def loss_and_grads(lay, mlp, grid, samples, fp16=True):
+    """Loss and (mlp grad (N_PARAMS,), grid grad (LEVELS, T, 2))."""
+    q = samples
+    y = samples[:, TARGET:TARGET + 3].astype(np.float32 if fp16 else np.float64)
+    n = len(q)
+    p, acts, W = forward(lay, mlp, grid, q, fp16)
+    den = p * p + 0.01
+    loss = float(np.sum((p - y) ** 2 / den) / (3 * n))
+    g = np.zeros((n, 16), p.dtype)
+    g[:, :3] = 2.0 * (p - y) / den                 # scaled by 3n, as on the device
+    g = _h(g, fp16)
+    grads = []
+    dW5 = g.T @ acts[5]
+    grads.append(dW5)
+    dz = g @ W[5]
+    dz = _h(np.where(acts[5] > 0, dz, 0), fp16)
+    for j in range(4, -1, -1):
+        grads.append(dz.T @ acts[j])
+        dx = dz @ W[j]
+        if j > 0:
+            dz = _h(np.where(acts[j] > 0, dx, 0), fp16)
+    grads = grads[::-1]            # W0..W4, W5
+    scale = 1.0 / (3 * n)
+    gm = np.concatenate([grads[j].reshape(-1) for j in range(5)] + [grads[5].reshape(-1)]) * scale
+    gm[5 * 4096 + 3 * 64:] = 0.0
+    # grid: dx are the input gradients of the last layer step (j = 0)
+    gg = np.zeros((LEVELS, lay.T, 2), np.float64)
+    pu = lay.unit(q[:, 0:3])
+    for l in range(LEVELS):
+        idx, w = lay.corners(pu, l)
+        for k in range(8):
+            np.add.at(gg[l, :, 0], idx[:, k], w[:, k] * dx[:, 2 * l])
+            np.add.at(gg[l, :, 1], idx[:, k], w[:, k] * dx[:, 2 * l + 1])
+    return loss, gm, gg * scale, p
+
+
+def adam(w, m, v, g, step, lr, b1=0.9, b2=0.99, eps=1e-10):
+    """One Adam step (Kingma and Ba 2015, Algorithm 1) on float64 copies."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    mh = m / (1 - b1 ** step)
+    vh = v / (1 - b2 ** step)
+    return w - lr * mh / (np.sqrt(vh) + eps), m, v

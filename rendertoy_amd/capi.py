@@ -69,6 +69,17 @@ class rt_stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class rt_cache_params(C.Structure):
+    _fields_ = [("log2_table", C.c_uint32), ("base_resolution", C.c_uint32), ("max_resolution", C.c_uint32),
+                ("train_batch", C.c_uint32), ("train_paths", C.c_uint32), ("first_vertex", C.c_uint32),
+                ("lr_mlp", C.c_float), ("lr_grid", C.c_float), ("target_clamp", C.c_float), ("seed", C.c_uint64)]
+
+
+CACHE_LEVELS = 16
+CACHE_MLP_PARAMS = 5 * 4096 + 16 * 64
+CACHE_RECORD = 20          # floats per cache query / training sample
+CACHE_BOUNCES, CACHE_TARGET = 13, 16
+
 _lib = None
 
 
@@ -97,6 +108,16 @@ def lib():
     L.rt_scene_device_output.argtypes = [vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.c_size_t)]
     L.rt_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
     L.rt_intersect.argtypes = [vp, C.c_uint32, vp, vp, vp, vp]
+    L.rt_cache_create.argtypes = [vp, C.POINTER(rt_cache_params), C.POINTER(vp)]
+    L.rt_cache_destroy.argtypes = [vp]
+    L.rt_cache_train.argtypes = [vp, C.POINTER(rt_camera), C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]
+    L.rt_cache_train_samples.argtypes = [vp, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_float)]
+    L.rt_cache_eval.argtypes = [vp, C.c_uint32, vp, vp]
+    L.rt_cache_grad.argtypes = [vp, C.c_uint32, vp, C.POINTER(C.c_float), vp, vp]
+    L.rt_cache_get_params.argtypes = [vp, vp, vp]
+    L.rt_cache_set_params.argtypes = [vp, vp, vp]
+    L.rt_cache_layout.argtypes = [vp, vp, vp, vp, vp]
+    L.rt_render_pt_cached.argtypes = [vp, vp, C.POINTER(rt_camera), C.POINTER(rt_pt_params), vp]
     _lib = L
     return L
 
@@ -208,4 +229,103 @@ class Scene:
         p = rt_pt_params(spp or self.sc.spp, max_depth or self.sc.max_depth, seed, batch_paths, flags)
         rgb = np.zeros((cam.width * cam.height, 3), np.float32) if download else None
         check(lib().rt_render_pt(self.h, C.byref(cam), C.byref(p), rgb.ctypes.data if download else None))
+        return rgb
+
+
+class Cache:
+    """An rt_cache (radiance cache) over a Scene."""
+
+    DEFAULTS = dict(log2_table=19, base_resolution=16, max_resolution=2048, train_batch=65536,
+                    train_paths=65536, first_vertex=1, lr_mlp=1e-3, lr_grid=1e-2, target_clamp=100.0, seed=1)
+
+    def __init__(self, scene: Scene, **kw):
+        prm = dict(self.DEFAULTS)
+        unknown = set(kw) - set(prm)
+        if unknown:
+            raise TypeError("unknown cache parameters %s" % sorted(unknown))
+        prm.update(kw)
+        self.prm = prm
+        self.scene = scene
+        p = rt_cache_params(**prm)
+        h = C.c_void_p()
+        check(lib().rt_cache_create(scene.h, C.byref(p), C.byref(h)))
+        self.h = h
+        self.table = 1 << prm["log2_table"]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_cache_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layout(self):
+        res = np.zeros(CACHE_LEVELS, np.float32)
+        dense = np.zeros(CACHE_LEVELS, np.uint32)
+        bmin = np.zeros(3, np.float32)
+        binv = np.zeros(3, np.float32)
+        check(lib().rt_cache_layout(self.h, res.ctypes.data, dense.ctypes.data, bmin.ctypes.data, binv.ctypes.data))
+        return dict(res=res, dense=dense, bmin=bmin, binv=binv, log2_table=self.prm["log2_table"])
+
+    def get_params(self, grid=True):
+        mlp = np.zeros(CACHE_MLP_PARAMS, np.float32)
+        g = np.zeros((CACHE_LEVELS, self.table, 2), np.float32) if grid else None
+        check(lib().rt_cache_get_params(self.h, mlp.ctypes.data, g.ctypes.data if grid else None))
+        return mlp, g
+
+    def set_params(self, mlp=None, grid=None):
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        if m is not None and m.size != CACHE_MLP_PARAMS:
+            raise ValueError("mlp must hold %d floats" % CACHE_MLP_PARAMS)
+        if g is not None and g.size != CACHE_LEVELS * self.table * 2:
+            raise ValueError("grid must hold levels x table x 2 floats")
+        check(lib().rt_cache_set_params(self.h, None if m is None else m.ctypes.data,
+                                        None if g is None else g.ctypes.data))
+
+    @staticmethod
+    def _records(a):
+        a = np.ascontiguousarray(a, np.float32)
+        if a.ndim != 2 or a.shape[1] != CACHE_RECORD:
+            raise ValueError("records are n x %d floats" % CACHE_RECORD)
+        return a
+
+    def eval(self, queries):
+        q = self._records(queries)
+        out = np.zeros((len(q), 4), np.float32)
+        check(lib().rt_cache_eval(self.h, len(q), q.ctypes.data, out.ctypes.data))
+        return out[:, :3]
+
+    def grad(self, samples, grid=True):
+        s = self._records(samples)
+        loss = C.c_float()
+        gm = np.zeros(CACHE_MLP_PARAMS, np.float32)
+        gg = np.zeros((CACHE_LEVELS, self.table, 2), np.float32) if grid else None
+        check(lib().rt_cache_grad(self.h, len(s), s.ctypes.data, C.byref(loss), gm.ctypes.data,
+                                  gg.ctypes.data if grid else None))
+        return loss.value, gm, gg
+
+    def train_samples(self, samples, steps=1):
+        s = self._records(samples)
+        loss = C.c_float()
+        check(lib().rt_cache_train_samples(self.h, len(s), s.ctypes.data, steps, C.byref(loss)))
+        return loss.value
+
+    def train(self, W=None, H=None, max_depth=None, steps=1):
+        cam = self.scene.camera(W, H)
+        loss = C.c_float()
+        check(lib().rt_cache_train(self.h, C.byref(cam), max_depth or self.scene.sc.max_depth, steps,
+                                   C.byref(loss)))
+        return loss.value
+
+    def render_pt(self, W=None, H=None, spp=None, max_depth=None, seed=0, batch_paths=0, flags=0, download=True):
+        cam = self.scene.camera(W, H)
+        p = rt_pt_params(spp or self.scene.sc.spp, max_depth or self.scene.sc.max_depth, seed, batch_paths, flags)
+        rgb = np.zeros((cam.width * cam.height, 3), np.float32) if download else None
+        check(lib().rt_render_pt_cached(self.scene.h, self.h, C.byref(cam), C.byref(p),
+                                        rgb.ctypes.data if download else None))
         return rgb

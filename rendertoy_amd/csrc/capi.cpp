@@ -221,6 +221,15 @@ int rt_scene_create(const rt_scene_desc *desc, rt_scene **out) {
             c[6] = e2.x; c[7] = e2.y; c[8] = e2.z;
             geo[i] = make_float4(nrm.x, nrm.y, nrm.z, area);
         }
+        for (int k = 0; k < 3; ++k) {
+            s->bbox[k] = n ? 1e30f : 0.0f;
+            s->bbox[3 + k] = n ? -1e30f : 1.0f;
+        }
+        for (size_t i = 0; i < (size_t)n * 3; ++i)
+            for (int k = 0; k < 3; ++k) {
+                s->bbox[k] = std::min(s->bbox[k], desc->vertices[3 * i + k]);
+                s->bbox[3 + k] = std::max(s->bbox[3 + k], desc->vertices[3 * i + k]);
+            }
         Bvh4 bvh = build_bvh4(n, cache.data());
         std::vector<float4> tris((size_t)3 * (n ? n : 1));
         for (uint32_t k = 0; k < n; ++k) {
@@ -295,6 +304,7 @@ int rt_scene_destroy(rt_scene *scene) {
     return guarded([&]() {
         if (scene) {
             (void)hipStreamSynchronize(scene->stream);
+            detach_caches(scene);
             delete scene;
         }
         return (int)RT_OK;

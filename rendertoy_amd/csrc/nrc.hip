@@ -1,0 +1,726 @@
+// nrc.hip -- online-trained neural radiance cache (no counterpart in RenderToy).
+//
+// Encoding: multiresolution hash grid over the scene box, 16 levels x 2^19
+// entries x 2 fp16 features (levels whose (R+1)^3 grid fits the table are
+// indexed densely), real spherical harmonics up to degree 3 (16 values) of the
+// outgoing direction, the shading normal mapped to [0,1], roughness and
+// albedo, and the bounces left after the vertex (the estimator's depth cut
+// makes the radiance it returns depend on them): 56 inputs padded to 64.
+// Network: 64 -> 64 (x5, ReLU) -> 3 (padded to 16), no biases, fp16 weights
+// and activations on MFMA 16x16x16 tiles with fp32 accumulation.
+//
+// MFMA operand layout (v_mfma_f32_16x16x16f16, wave64, checked on gfx950):
+//   A: lane l holds A[l%16][4*(l/16)+i]   B: lane l holds B[4*(l/16)+i][l%16]
+//   D: lane l holds D[4*(l/16)+i][l%16]
+// A layer's D tile is therefore the next layer's B tile as is: activations
+// stay in registers from layer to layer; weights are read from LDS.
+// Each wave owns 64 queries (4 N-tiles of 16); lane group g = l/16 encodes
+// input features 16s + 4g + i (s = 0..3) of query l%16 of each N-tile.
+//
+// Training (per step, B samples): forward stores activations (fp16, per wave
+// [feature][sample]) and the loss gradient; backward chains dZ through W^T in
+// registers, forms per-wave dW partials with MFMA over the 64 samples (A and
+// B read as 4 consecutive samples of one feature), and scatter-adds the grid
+// gradient; Adam then reduces the partials in a fixed order (deterministic
+// MLP gradients) and updates the fp32 masters and their fp16 copies.  Loss:
+// relative L2, (p - y)^2 / (p^2 + 0.01), with the gradient scaled by 3B to
+// stay in fp16 range and unscaled in Adam.
+#include "dev_bsdf.h"
+#include "nrc_kernels.h"
+
+namespace rtamd {
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kStride = 68;            // LDS row stride (halves) of a weight matrix
+constexpr int kLdsW = 5 * 64 * kStride + 16 * kStride;   // forward: W_j rows = outputs
+constexpr int kLdsWT = 6 * 64 * kStride;                  // backward: W_j^T rows = inputs
+
+__device__ __forceinline__ f4 mfma(h4 a, h4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0); }
+
+// ---------------------------------------------------------------- encoding
+
+__device__ __forceinline__ uint32_t grid_index(const NrcGrid &g, int l, uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t idx;
+    if (g.dense[l]) {
+        const uint32_t s = g.side[l];
+        idx = x + s * (y + s * z);
+    } else {
+        idx = x ^ (y * 2654435761u) ^ (z * 805459861u);
+    }
+    return idx & g.tmask;
+}
+
+// Position in the unit cube of the scene box.
+__device__ __forceinline__ void unit_pos(const NrcGrid &g, const float *pos, float p[3]) {
+    for (int k = 0; k < 3; ++k) {
+        float u = (pos[k] - g.bmin[k]) * g.binv[k];
+        p[k] = fminf(fmaxf(u, 0.0f), 1.0f);
+    }
+}
+
+// Trilinear corner c (0..7) of level l: table entry and weight.
+struct Corners {
+    uint32_t i0[3];
+    float f[3];
+};
+__device__ __forceinline__ Corners level_cell(const NrcGrid &g, int l, const float p[3]) {
+    Corners c;
+    const float R = g.res[l];
+    for (int k = 0; k < 3; ++k) {
+        float x = p[k] * R;
+        float fl = floorf(x);
+        fl = fminf(fl, R - 1.0f);
+        c.i0[k] = (uint32_t)fl;
+        c.f[k] = x - fl;
+    }
+    return c;
+}
+__device__ __forceinline__ void corner(const NrcGrid &g, int l, const Corners &c, int k, uint32_t &idx, float &w) {
+    const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = (k >> 2) & 1;
+    idx = grid_index(g, l, c.i0[0] + dx, c.i0[1] + dy, c.i0[2] + dz);
+    w = (dx ? c.f[0] : 1.0f - c.f[0]) * (dy ? c.f[1] : 1.0f - c.f[1]) * (dz ? c.f[2] : 1.0f - c.f[2]);
+}
+
+__device__ __forceinline__ void level_features(const NrcGrid &g, int l, const float p[3], float &a, float &b) {
+    const Corners c = level_cell(g, l, p);
+    const __half2 *tab = reinterpret_cast<const __half2 *>(g.t16) + (size_t)l * (g.tmask + 1);
+    a = 0.0f;
+    b = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint32_t idx;
+        float w;
+        corner(g, l, c, k, idx, w);
+        const float2 v = __half22float2(tab[idx]);
+        a += w * v.x;
+        b += w * v.y;
+    }
+}
+
+// Real spherical harmonics of degree <= 3 (16 values) of a unit direction.
+__device__ __forceinline__ void sh16(float x, float y, float z, float o[16]) {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    o[0] = 0.28209479177387814f;
+    o[1] = -0.48860251190291987f * y;
+    o[2] = 0.48860251190291987f * z;
+    o[3] = -0.48860251190291987f * x;
+    o[4] = 1.0925484305920792f * x * y;
+    o[5] = -1.0925484305920792f * y * z;
+    o[6] = 0.94617469575755997f * zz - 0.31539156525251999f;
+    o[7] = -1.0925484305920792f * x * z;
+    o[8] = 0.54627421529603959f * (xx - yy);
+    o[9] = 0.59004358992664352f * y * (-3.0f * xx + yy);
+    o[10] = 2.8906114426405538f * x * y * z;
+    o[11] = 0.45704579946446572f * y * (1.0f - 5.0f * zz);
+    o[12] = 0.3731763325901154f * z * (5.0f * zz - 3.0f);
+    o[13] = 0.45704579946446572f * x * (1.0f - 5.0f * zz);
+    o[14] = 1.4453057213202769f * z * (xx - yy);
+    o[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
+}
+
+// Input features 16s + 4g + i of one query, as the four B tiles of lane group g.
+__device__ __forceinline__ void encode(const NrcGrid &g, const float *q, uint32_t grp, h4 in[4]) {
+    float p[3];
+    unit_pos(g, q, p);
+    float a, b, c, d;
+    level_features(g, 2 * grp, p, a, b);
+    level_features(g, 2 * grp + 1, p, c, d);
+    in[0] = h4{(_Float16)a, (_Float16)b, (_Float16)c, (_Float16)d};
+    level_features(g, 8 + 2 * grp, p, a, b);
+    level_features(g, 8 + 2 * grp + 1, p, c, d);
+    in[1] = h4{(_Float16)a, (_Float16)b, (_Float16)c, (_Float16)d};
+    float sh[16];
+    sh16(q[3], q[4], q[5], sh);
+    in[2] = h4{(_Float16)sh[4 * grp], (_Float16)sh[4 * grp + 1], (_Float16)sh[4 * grp + 2], (_Float16)sh[4 * grp + 3]};
+    if (grp == 0)
+        in[3] = h4{(_Float16)(q[6] * 0.5f + 0.5f), (_Float16)(q[7] * 0.5f + 0.5f), (_Float16)(q[8] * 0.5f + 0.5f),
+                   (_Float16)q[9]};
+    else if (grp == 1)
+        in[3] = h4{(_Float16)q[10], (_Float16)q[11], (_Float16)q[12], (_Float16)(q[kRecBounces] * kNrcBounceScale)};
+    else
+        in[3] = h4{(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
+}
+
+// ---------------------------------------------------------------- MLP forward
+
+__device__ __forceinline__ void load_weights(const _Float16 *w16, _Float16 *lds) {
+    // W_j (rows = outputs) into LDS rows of kStride halves
+    for (int e = threadIdx.x; e < kNrcParams; e += blockDim.x) {
+        const int j = e < 5 * 4096 ? e / 4096 : 5;
+        const int r = (e - j * 4096) / 64, c = (e - j * 4096) % 64;
+        lds[(j * 64 + r) * kStride + c] = w16[e];
+    }
+}
+
+__device__ __forceinline__ h4 lds_a(const _Float16 *lds, int j, int t, int s, uint32_t lane) {
+    const _Float16 *row = lds + (j * 64 + 16 * t + (lane & 15)) * kStride + 16 * s + 4 * (lane >> 4);
+    return *reinterpret_cast<const h4 *>(row);
+}
+
+__device__ __forceinline__ h4 relu_h4(f4 v) {
+    return h4{(_Float16)fmaxf(v[0], 0.0f), (_Float16)fmaxf(v[1], 0.0f), (_Float16)fmaxf(v[2], 0.0f),
+              (_Float16)fmaxf(v[3], 0.0f)};
+}
+
+// 5 hidden layers on act[nt][s] in place; returns the output tiles out[nt].
+__device__ __forceinline__ void mlp_forward(const _Float16 *lds, uint32_t lane, h4 act[4][4], f4 out[4],
+                                            _Float16 *store = nullptr, int64_t store_stride = 0) {
+#pragma unroll 1
+    for (int j = 0; j < 5; ++j) {
+        f4 acc[4][4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[t][nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const h4 a = lds_a(lds, j, t, s, lane);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) acc[t][nt] = mfma(a, act[nt][s], acc[t][nt]);
+            }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) act[nt][t] = relu_h4(acc[t][nt]);
+        if (store) {
+            // activation j+1 as [feature][sample] of this wave
+            _Float16 *dst = store + (int64_t)(j + 1) * store_stride;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        dst[(16 * t + 4 * (lane >> 4) + i) * 64 + 16 * nt + (lane & 15)] = act[nt][t][i];
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f4 acc = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = mfma(lds_a(lds, 5, 0, s, lane), act[nt][s], acc);
+        out[nt] = acc;
+    }
+}
+
+__device__ __forceinline__ void store_input(_Float16 *dst, uint32_t lane, h4 act[4][4]) {
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) dst[(16 * s + 4 * (lane >> 4) + i) * 64 + 16 * nt + (lane & 15)] = act[nt][s][i];
+}
+
+// Inference.  mode 0: out[q] = prediction (3 floats of 4); mode 1: the path
+// slot in the query record receives rad[slot] += thr[slot] * max(pred, 0).
+template <int MODE>
+__global__ __launch_bounds__(256) void nrc_infer_kernel(NrcGrid g, const _Float16 *w16, const float *queries,
+                                                        const uint32_t *count_ptr, uint32_t count, float4 *out,
+                                                        float4 *rad, const float4 *thr) {
+    __shared__ _Float16 lds[kLdsW];
+    load_weights(w16, lds);
+    __syncthreads();
+    const uint32_t n = count_ptr ? *count_ptr : count;
+    const uint32_t lane = threadIdx.x & 63u, grp = lane >> 4;
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); (size_t)w * 64 < n; w += waves) {
+        h4 act[4][4];
+        uint32_t qi[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            qi[nt] = w * 64 + 16 * nt + (lane & 15);
+            if (qi[nt] < n) {
+                encode(g, queries + kNrcFloats * (size_t)qi[nt], grp, act[nt]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) act[nt][s] = h4{0, 0, 0, 0};
+            }
+        }
+        f4 o[4];
+        mlp_forward(lds, lane, act, o);
+        if (grp == 0) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                if (qi[nt] >= n) continue;
+                if (MODE == 0) {
+                    out[qi[nt]] = make_float4(o[nt][0], o[nt][1], o[nt][2], 0.0f);
+                } else {
+                    const uint32_t slot = __float_as_uint(queries[kNrcFloats * (size_t)qi[nt] + kRecSlot]);
+                    const float4 t = thr[slot];
+                    float4 l = rad[slot];
+                    l.x += t.x * fmaxf(o[nt][0], 0.0f);
+                    l.y += t.y * fmaxf(o[nt][1], 0.0f);
+                    l.z += t.z * fmaxf(o[nt][2], 0.0f);
+                    rad[slot] = l;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- training
+
+// Forward over n sample records.  act: per
+// wave, 6 activations of [64 features][64 samples] fp16; dout: per wave
+// [16][64] loss-gradient tile (scaled by 3n); loss: per wave sum.
+__global__ __launch_bounds__(256) void nrc_train_fwd_kernel(NrcGrid g, const _Float16 *w16, const float *samples,
+                                                            uint32_t n, _Float16 *act, _Float16 *dout,
+                                                            float *loss_part, float *pred_out) {
+    __shared__ _Float16 lds[kLdsW];
+    load_weights(w16, lds);
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, grp = lane >> 4;
+    const uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if ((size_t)w * 64 >= ((size_t)n + 63) / 64 * 64) return;
+    h4 a[4][4];
+    uint32_t qi[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        qi[nt] = w * 64 + 16 * nt + (lane & 15);
+        if (qi[nt] < n) {
+            encode(g, samples + kNrcFloats * (size_t)qi[nt], grp, a[nt]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) a[nt][s] = h4{0, 0, 0, 0};
+        }
+    }
+    const int64_t stride = 64 * 64;
+    _Float16 *mine = act + (int64_t)w * 6 * stride;
+    store_input(mine, lane, a);
+    f4 o[4];
+    mlp_forward(lds, lane, a, o, mine, stride);
+    float loss = 0.0f;
+    _Float16 *dmine = dout + (int64_t)w * 16 * 64;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float gval = 0.0f;
+            const int f = 4 * grp + i;
+            if (f < 3 && qi[nt] < n) {
+                const float pr = o[nt][i];
+                const float y = samples[kNrcFloats * (size_t)qi[nt] + kRecTarget + f];
+                const float den = pr * pr + 0.01f;
+                loss += (pr - y) * (pr - y) / den;
+                gval = 2.0f * (pr - y) / den;       // d loss / d pred, scaled by 3n
+                if (pred_out) pred_out[4 * (size_t)qi[nt] + f] = pr;
+            }
+            dmine[f * 64 + 16 * nt + (lane & 15)] = (_Float16)gval;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) loss_part[w] = loss;
+}
+
+__device__ __forceinline__ h4 gl_b(const _Float16 *tile, int f0, int ks, uint32_t lane) {
+    // rows = features (64 samples each): feature f0 + lane%16, samples 16ks + 4(lane/16) ..+3
+    return *reinterpret_cast<const h4 *>(tile + (f0 + (lane & 15)) * 64 + 16 * ks + 4 * (lane >> 4));
+}
+
+// Backward.  wpart: per wave, kNrcParams fp32 partial weight gradients (scaled).
+// grid_grad: fp32 table gradients (scaled), scatter-added.
+__global__ __launch_bounds__(256) void nrc_train_bwd_kernel(NrcGrid g, const _Float16 *wt16, const float *samples,
+                                                            uint32_t n, const _Float16 *act, const _Float16 *dout,
+                                                            float *wpart) {
+    __shared__ _Float16 lds[kLdsWT];
+    __shared__ _Float16 dz_tile[4][64 * 64];
+    // W_j^T (rows = inputs) into LDS
+    for (int e = threadIdx.x; e < kNrcParams; e += blockDim.x) {
+        const int j = e < 5 * 4096 ? e / 4096 : 5;
+        const int cols = j < 5 ? 64 : 16;
+        const int r = (e - j * 4096) / cols, c = (e - j * 4096) % cols;
+        lds[(j * 64 + r) * kStride + c] = wt16[e];
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, grp = lane >> 4, wv = threadIdx.x >> 6;
+    const uint32_t w = blockIdx.x * (blockDim.x >> 6) + wv;
+    const bool live = (size_t)w * 64 < n;
+    const int64_t stride = 64 * 64;
+    const _Float16 *mine = act + (int64_t)w * 6 * stride;
+    const _Float16 *dmine = dout + (int64_t)w * 16 * 64;
+    float *part = wpart + (int64_t)w * kNrcParams;
+    _Float16 *dzt = dz_tile[wv];
+
+    // --- output layer (j = 5): dW5[16 x 64] and dX5 = W5^T dOut
+    h4 dz[4][4];   // B tiles of the current layer's dZ: [k-step][n-tile]
+    if (live) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            f4 acc = f4{0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) acc = mfma(gl_b(dmine, 0, ks, lane), gl_b(mine + 5 * stride, 16 * u, ks, lane), acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) part[5 * 4096 + (4 * grp + i) * 64 + 16 * u + (lane & 15)] = acc[i];
+        }
+        // dOut as the single K step of W5^T dOut
+        h4 d5[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) d5[nt][i] = dmine[(4 * grp + i) * 64 + 16 * nt + (lane & 15)];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const h4 a = *reinterpret_cast<const h4 *>(lds + (5 * 64 + 16 * t + (lane & 15)) * kStride + 4 * grp);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const f4 x = mfma(a, d5[nt], f4{0, 0, 0, 0});
+                // ReLU mask of activation 5 (input of layer 5)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const _Float16 y = mine[5 * stride + (16 * t + 4 * grp + i) * 64 + 16 * nt + (lane & 15)];
+                    dz[t][nt][i] = (_Float16)(y > (_Float16)0.0f ? x[i] : 0.0f);
+                }
+            }
+        }
+    }
+    // --- hidden layers j = 4 .. 0
+#pragma unroll 1
+    for (int j = 4; j >= 0; --j) {
+        if (live) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) dzt[(16 * t + 4 * grp + i) * 64 + 16 * nt + (lane & 15)] = dz[t][nt][i];
+        }
+        __syncthreads();
+        if (live) {
+            // dW_j = dZ_j X_j^T over this wave's 64 samples
+            const _Float16 *x = mine + (int64_t)j * stride;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                h4 at[4];
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) at[ks] = gl_b(dzt, 16 * t, ks, lane);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    f4 acc = f4{0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) acc = mfma(at[ks], gl_b(x, 16 * u, ks, lane), acc);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        part[j * 4096 + (16 * t + 4 * grp + i) * 64 + 16 * u + (lane & 15)] = acc[i];
+                }
+            }
+            // dX_j = W_j^T dZ_j
+            f4 dx[4][4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) dx[t][nt] = f4{0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const h4 a = *reinterpret_cast<const h4 *>(lds + (j * 64 + 16 * t + (lane & 15)) * kStride + 16 * s +
+                                                              4 * grp);
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt) dx[t][nt] = mfma(a, dz[s][nt], dx[t][nt]);
+                }
+            if (j > 0) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const _Float16 y = x[(16 * t + 4 * grp + i) * 64 + 16 * nt + (lane & 15)];
+                            dz[t][nt][i] = (_Float16)(y > (_Float16)0.0f ? dx[t][nt][i] : 0.0f);
+                        }
+            } else {
+                // input gradient of the grid features (t = 0: levels 2g, 2g+1; t = 1: 8+2g, 9+2g)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    const uint32_t q = w * 64 + 16 * nt + (lane & 15);
+                    if (q >= n) continue;
+                    float p[3];
+                    unit_pos(g, samples + kNrcFloats * (size_t)q, p);
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int l = 8 * t + 2 * grp + h;
+                            const float ga = dx[t][nt][2 * h], gb = dx[t][nt][2 * h + 1];
+                            if (ga == 0.0f && gb == 0.0f) continue;
+                            const Corners c = level_cell(g, l, p);
+                            float *gg = g.grad + (size_t)l * (g.tmask + 1) * 2;
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                uint32_t idx;
+                                float wgt;
+                                corner(g, l, c, k, idx, wgt);
+                                atomicAdd(gg + 2 * idx, wgt * ga);
+                                atomicAdd(gg + 2 * idx + 1, wgt * gb);
+                            }
+                        }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Adam on the MLP: reduce the per-wave partials in wave order (deterministic).
+__global__ void nrc_adam_mlp_kernel(float *w, float *m, float *v, _Float16 *w16, _Float16 *wt16, const float *wpart,
+                                    uint32_t nwaves, float inv_scale, float lr, float b1, float b2, float eps,
+                                    float c1, float c2) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)kNrcParams) return;
+    const int j = p < 5 * 4096 ? p / 4096 : 5;
+    const int cols = 64;
+    const int r = (p - j * 4096) / cols, c = (p - j * 4096) % cols;
+    if (j == 5 && r >= 3) {   // padded output rows stay zero
+        w16[p] = (_Float16)0.0f;
+        wt16[5 * 4096 + c * 16 + r] = (_Float16)0.0f;
+        return;
+    }
+    float gsum = 0.0f;
+    for (uint32_t k = 0; k < nwaves; ++k) gsum += wpart[(size_t)k * kNrcParams + p];
+    const float gr = gsum * inv_scale;
+    const float mm = b1 * m[p] + (1.0f - b1) * gr;
+    const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
+    m[p] = mm;
+    v[p] = vv;
+    const float nw = w[p] - lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+    w[p] = nw;
+    w16[p] = (_Float16)nw;
+    if (j < 5) wt16[j * 4096 + c * 64 + r] = (_Float16)nw;
+    else wt16[5 * 4096 + c * 16 + r] = (_Float16)nw;
+}
+
+// Adam on the grid; clears the gradient it consumed.
+__global__ void nrc_adam_grid_kernel(float *w, float *m, float *v, _Float16 *w16, float *grad, size_t n,
+                                     float inv_scale, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float gr = grad[p] * inv_scale;
+    grad[p] = 0.0f;
+    const float mm = b1 * m[p] + (1.0f - b1) * gr;
+    const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
+    m[p] = mm;
+    v[p] = vv;
+    const float nw = w[p] - lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+    w[p] = nw;
+    w16[p] = (_Float16)nw;
+}
+
+__global__ void nrc_f32_to_f16_kernel(const float *src, _Float16 *dst, size_t n) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = (_Float16)src[p];
+}
+
+__global__ void nrc_transpose_kernel(const float *w, _Float16 *wt16) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)kNrcParams) return;
+    const int j = p < 5 * 4096 ? p / 4096 : 5;
+    const int r = (p - j * 4096) / 64, c = (p - j * 4096) % 64;
+    if (j < 5) wt16[j * 4096 + c * 64 + r] = (_Float16)w[p];
+    else if (r < 16) wt16[5 * 4096 + c * 16 + r] = (_Float16)w[p];
+}
+
+__global__ void nrc_zero_kernel(float *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+// ---------------------------------------------------------------- training paths
+
+// Plain paths (the renderer's estimator) from random pixels; each vertex k >=
+// first_vertex becomes a sample: its features and the radiance the renderer
+// adds after vertex k's emission term, dl_k + w_k (e_{k+1} + R_{k+1}).
+__global__ __launch_bounds__(256) void nrc_gen_kernel(DevScene sc, NrcGen gp, float *out, uint32_t *count,
+                                                      uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
+    __shared__ uint32_t s_ref[kLdsStack * 256];
+    __shared__ float s_t[kLdsStack * 256];
+    const uint32_t tid = threadIdx.x;
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
+    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    uint32_t nn = 0, ntt = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t pth = (uint32_t)gtid; pth < gp.npaths; pth += stride) {
+        const uint4 r0 = philox(make_uint4(pth, gp.frame, 0xFFFFFFF0u, 0u), gp.key0, gp.key1);
+        const uint32_t pixel = pick(r0.x, gp.npix);
+        const uint32_t sample = r0.y;   // a sample stream disjoint from rendering in practice
+        V3 o, d;
+        camera_ray(gp.cam, gp.divW, gp.divH, pixel % gp.cam.width, pixel / gp.cam.width, o, d);
+        float feat[kNrcMaxV][13];
+        V3 ev[kNrcMaxV + 1], dlv[kNrcMaxV], wv[kNrcMaxV];
+        int nv = 0;
+        V3 absn = v3(0.0f);
+        float last_pdf = 0.0f;
+        uint32_t excl = kNone;
+        for (uint32_t dep = 0; dep < gp.max_depth; ++dep) {
+            Hit h;
+            trace<false, false>(sc, o, d, excl, kNone, 0.0f, stk, h, nn, ntt);
+            if (h.id == kNone) {
+                ev[dep] = (-d.z < 0.0f) ? ld3(sc.sky) : ld3(sc.ground);
+                break;
+            }
+            const uint4 ra = philox(make_uint4(pixel, 0x80000000u | (sample & 0x7FFFFFFFu), dep, 0u), gp.key0,
+                                    gp.key1);
+            const Mat m = load_mat(sc.mats[sc.info[h.id] & 0x7FFFFFFFu]);
+            const V3 pos = add(o, muls(d, h.t));
+            const V3 N = shading_normal(sc, h.id, h.u, h.v);
+            V3 ff = N;
+            float eta;
+            if (dot(d, ff) > 0.0f) {
+                ff = neg(ff);
+                eta = m.ior;
+                absn = v3(0.0f);
+            } else {
+                eta = 1.0f / m.ior;
+            }
+            const float pdf_e = emission_pdf(pos, o, neg(d), N, sc.geo[h.id].w);
+            ev[dep] = dep == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
+            const float len = length(sub(pos, o));
+            const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
+            float *f = feat[dep];
+            f[0] = pos.x; f[1] = pos.y; f[2] = pos.z;
+            f[3] = -d.x; f[4] = -d.y; f[5] = -d.z;
+            f[6] = ff.x; f[7] = ff.y; f[8] = ff.z;
+            f[9] = m.roughness;
+            f[10] = m.base.x; f[11] = m.base.y; f[12] = m.base.z;
+            V3 dl = v3(0.0f);
+            if (sc.num_emit > 0) {
+                const EmitterRec &E = sc.emit[pick(ra.x, sc.num_emit)];
+                const float sqr1 = sqrtf(u01(ra.y));
+                const float r2 = u01(ra.z);
+                const V3 epos = add(add(muls(ld3(E.e1), 1.0f - sqr1), muls(ld3(E.e2), (1.0f - r2) * sqr1)), ld3(E.v0));
+                const V3 dir_to = normalized(sub(epos, pos));
+                float te = 0.0f, ue = 0.0f, ve = 0.0f, tmax = kFloatInfinity;
+                if (E.tri != h.id && mt_intersect(pos, dir_to, ld3(E.v0), ld3(E.e1), ld3(E.e2), te, ue, ve) &&
+                    te < kFloatInfinity)
+                    tmax = te;
+                else
+                    ue = ve = 0.0f;
+                const bool esmooth = (sc.info[E.tri] & 0x80000000u) != 0u;
+                const V3 en = esmooth ? shading_normal(sc, E.tri, ue, ve) : ld3(E.normal);
+                const float lightpdf = emission_pdf(epos, pos, neg(dir_to), en, E.area);
+                float bsdfpdf;
+                const V3 fb = BsdfEval(m, eta, neg(d), ff, dir_to, bsdfpdf);
+                if (bsdfpdf > 0.0f) {
+                    Hit sh;
+                    const bool occ = trace<true, false>(sc, pos, dir_to, h.id, E.tri, tmax, stk, sh, nn, ntt);
+                    if (!occ) {
+                        const V3 em = ld3(sc.mats[sc.info[E.tri] & 0x7FFFFFFFu].emission);
+                        dl = mulv(divs(muls(mulv(muls(fb, PowerHeuristic(lightpdf, bsdfpdf)), em), (float)sc.num_emit),
+                                       lightpdf), br);
+                    }
+                }
+            }
+            dlv[dep] = dl;
+            nv = dep + 1;
+            ev[dep + 1] = v3(0.0f);
+            wv[dep] = v3(0.0f);
+            if (dep + 1 >= gp.max_depth) break;
+            const uint4 rb = philox(make_uint4(pixel, 0x80000000u | (sample & 0x7FFFFFFFu), dep, 1u), gp.key0, gp.key1);
+            V3 nd, color;
+            float bpdf;
+            if (!BsdfSample(m, eta, ff, neg(d), u01(ra.w), u01(rb.x), nd, color, bpdf)) break;
+            absn = divs(neg(V3{logf(m.extinction.x), logf(m.extinction.y), logf(m.extinction.z)}), m.at_distance);
+            if (!(bpdf > 0.0f)) break;
+            wv[dep] = divs(mulv(br, color), bpdf);
+            o = pos;
+            d = nd;
+            excl = h.id;
+            last_pdf = bpdf;
+        }
+        // targets, last vertex first
+        V3 R = v3(0.0f);
+        float tgt[kNrcMaxV][3];
+        for (int k = nv - 1; k >= 0; --k) {
+            R = add(dlv[k], mulv(wv[k], add(ev[k + 1], R)));
+            tgt[k][0] = R.x; tgt[k][1] = R.y; tgt[k][2] = R.z;
+        }
+        const int first = (int)gp.first_vertex;
+        const int ns = nv > first ? nv - first : 0;
+        if (ns == 0) continue;
+        const uint32_t base = atomicAdd(count, (uint32_t)ns);
+        for (int k = first; k < nv; ++k) {
+            const uint32_t slot = base + (uint32_t)(k - first);
+            if (slot >= gp.capacity) break;
+            float *dst = out + kNrcFloats * (size_t)slot;
+            bool finite = isfinite(tgt[k][0]) && isfinite(tgt[k][1]) && isfinite(tgt[k][2]);
+            for (int c = 0; c < 13; ++c) dst[c] = feat[k][c];
+            dst[kRecBounces] = (float)(gp.max_depth - 1u - (uint32_t)k);
+            dst[kRecBounces + 1] = 0.0f;
+            dst[kRecSlot] = 0.0f;
+            for (int c = 0; c < 3; ++c) dst[kRecTarget + c] = finite ? fminf(tgt[k][c], gp.target_clamp) : 0.0f;
+            dst[kRecTarget + 3] = 0.0f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t nrc_launch_infer(const NrcGrid &g, const _Float16 *w16, const float *queries, const uint32_t *count_ptr,
+                            uint32_t count, float4 *out, float4 *rad, const float4 *thr, uint32_t grid, int mode,
+                            hipStream_t s) {
+    if (mode == 0)
+        hipLaunchKernelGGL(nrc_infer_kernel<0>, dim3(grid), dim3(256), 0, s, g, w16, queries, count_ptr, count, out,
+                           rad, thr);
+    else
+        hipLaunchKernelGGL(nrc_infer_kernel<1>, dim3(grid), dim3(256), 0, s, g, w16, queries, count_ptr, count, out,
+                           rad, thr);
+    return hipGetLastError();
+}
+
+hipError_t nrc_launch_train_step(const NrcGrid &g, NrcTrain &t, const float *samples, uint32_t n, hipStream_t s) {
+    const uint32_t waves = (n + 63) / 64;
+    const uint32_t blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL(nrc_train_fwd_kernel, dim3(blocks), dim3(256), 0, s, g, t.w16, samples, n, t.act, t.dout,
+                       t.loss_part, t.pred_out);
+    hipLaunchKernelGGL(nrc_train_bwd_kernel, dim3(blocks), dim3(256), 0, s, g, t.wt16, samples, n, t.act, t.dout,
+                       t.wpart);
+    return hipGetLastError();
+}
+
+hipError_t nrc_launch_adam(const NrcGrid &g, NrcTrain &t, uint32_t n, int step, float lr_mlp, float lr_grid,
+                           bool update_grid, hipStream_t s) {
+    const float b1 = 0.9f, b2 = 0.99f, eps = 1e-10f;
+    const float c1 = 1.0f / (1.0f - powf(b1, (float)step)), c2 = 1.0f / (1.0f - powf(b2, (float)step));
+    const float inv_scale = 1.0f / (3.0f * (float)n);
+    const uint32_t waves = (n + 63) / 64;
+    hipLaunchKernelGGL(nrc_adam_mlp_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, t.w, t.m, t.v, t.w16,
+                       t.wt16, t.wpart, waves, inv_scale, lr_mlp, b1, b2, eps, c1, c2);
+    if (update_grid) {
+        const size_t ng = g.grid_params;
+        hipLaunchKernelGGL(nrc_adam_grid_kernel, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, t.grid,
+                           t.grid_m, t.grid_v, g.t16, g.grad, ng, inv_scale, lr_grid, b1, b2, eps, c1, c2);
+    }
+    return hipGetLastError();
+}
+
+hipError_t nrc_launch_sync_weights(const NrcGrid &g, NrcTrain &t, hipStream_t s) {
+    hipLaunchKernelGGL(nrc_f32_to_f16_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, t.w, t.w16,
+                       (size_t)kNrcParams);
+    hipLaunchKernelGGL(nrc_transpose_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, t.w, t.wt16);
+    const size_t ng = g.grid_params;
+    hipLaunchKernelGGL(nrc_f32_to_f16_kernel, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, t.grid, g.t16, ng);
+    return hipGetLastError();
+}
+
+hipError_t nrc_launch_zero(float *p, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(nrc_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t nrc_launch_gen(const DevScene &sc, const NrcGen &gp, float *out, uint32_t *count, uint32_t grid,
+                          uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride, hipStream_t s) {
+    hipLaunchKernelGGL(nrc_gen_kernel, dim3(grid), dim3(256), 0, s, sc, gp, out, count, ovf_ref, ovf_t, ovf_stride);
+    return hipGetLastError();
+}
+
+int nrc_gen_blocks_per_cu() {
+    int nb = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nrc_gen_kernel, 256, 0);
+    return nb > 0 ? nb : 1;
+}
+
+}  // namespace rtamd

@@ -1,0 +1,358 @@
+// nrc_host.cpp -- radiance cache, host side: allocation, initialisation,
+// training steps and the C ABI (rt_cache_*).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nrc.h"
+#include "pt.h"
+
+namespace rtamd {
+namespace {
+
+void hipchk(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+void *alloc(rt_cache *c, size_t bytes) {
+    DevBuf b;
+    b.bytes = bytes ? bytes : 16;
+    hipchk(hipMalloc(&b.p, b.bytes), "hipMalloc");
+    c->bufs.push_back(b);
+    return b.p;
+}
+
+// Grows the per-batch training buffers to hold n samples.
+void ensure_batch(rt_cache *c, uint32_t n) {
+    if (n <= c->capacity) return;
+    const uint32_t waves = (n + 63) / 64;
+    NrcTrain &t = c->tr;
+    c->samples = (float *)alloc(c, (size_t)n * kNrcFloats * sizeof(float));
+    t.act = (_Float16 *)alloc(c, (size_t)waves * 6 * 64 * 64 * sizeof(_Float16));
+    t.dout = (_Float16 *)alloc(c, (size_t)waves * 16 * 64 * sizeof(_Float16));
+    t.loss_part = (float *)alloc(c, (size_t)waves * sizeof(float));
+    t.wpart = (float *)alloc(c, (size_t)waves * kNrcParams * sizeof(float));
+    t.pred_out = (float *)alloc(c, (size_t)n * 4 * sizeof(float));
+    c->capacity = n;
+}
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        set_error("host out of memory");
+        return RT_ERR_OOM;
+    } catch (const std::invalid_argument &e) {
+        set_error(e.what());
+        return RT_ERR_INVALID;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return RT_ERR_HIP;
+    }
+}
+
+// The scene a cache call runs on; a cache whose scene was destroyed only allows rt_cache_destroy.
+rt_scene *live_scene(const rt_cache *c) {
+    if (!c) throw std::invalid_argument("cache is NULL");
+    if (!c->scene) throw std::invalid_argument("the cache's scene was destroyed");
+    return c->scene;
+}
+
+float step_loss(rt_cache *c, uint32_t n) {
+    const uint32_t waves = (n + 63) / 64;
+    std::vector<float> lp(waves);
+    hipchk(hipMemcpyAsync(lp.data(), c->tr.loss_part, waves * sizeof(float), hipMemcpyDeviceToHost,
+                          c->scene->stream),
+           "hipMemcpyAsync");
+    hipchk(hipStreamSynchronize(c->scene->stream), "hipStreamSynchronize");
+    double s = 0.0;
+    for (float v : lp) s += v;
+    return (float)(s / (3.0 * n));
+}
+
+void train_steps(rt_cache *c, const float *dev_samples, uint32_t n, uint32_t steps, float *loss_out) {
+    hipStream_t st = c->scene->stream;
+    for (uint32_t k = 0; k < steps; ++k) {
+        hipchk(nrc_launch_train_step(c->grid, c->tr, dev_samples, n, st), "nrc train step");
+        c->step += 1;
+        hipchk(nrc_launch_adam(c->grid, c->tr, n, c->step, c->prm.lr_mlp, c->prm.lr_grid, true, st), "nrc adam");
+    }
+    if (loss_out) *loss_out = step_loss(c, n);
+}
+
+}  // namespace
+
+void detach_caches(rt_scene *scene) {
+    for (rt_cache *c : scene->caches) c->scene = nullptr;
+    scene->caches.clear();
+}
+
+}  // namespace rtamd
+
+using namespace rtamd;
+
+rt_cache::~rt_cache() {
+    for (auto &b : bufs) (void)hipFree(b.p);
+}
+
+extern "C" {
+
+int rt_cache_create(rt_scene *scene, const rt_cache_params *prm, rt_cache **out) {
+    if (!scene || !prm || !out) {
+        set_error("scene, params and out are required");
+        return RT_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (prm->log2_table < 8 || prm->log2_table > 24 || prm->base_resolution < 2 ||
+        prm->max_resolution < prm->base_resolution || prm->first_vertex < 1 || prm->train_batch == 0) {
+        set_error("bad cache parameters");
+        return RT_ERR_INVALID;
+    }
+    rt_cache *c = new rt_cache();
+    int rc = guarded([&]() {
+        hipchk(hipSetDevice(scene->device), "hipSetDevice");
+        c->scene = scene;
+        c->device = scene->device;
+        c->prm = *prm;
+        NrcGrid &g = c->grid;
+        const uint32_t T = 1u << prm->log2_table;
+        g.tmask = T - 1;
+        const double b = std::exp((std::log((double)prm->max_resolution) - std::log((double)prm->base_resolution)) /
+                                  (kNrcLevels - 1));
+        for (int l = 0; l < kNrcLevels; ++l) {
+            const double r = std::floor((double)prm->base_resolution * std::pow(b, (double)l) + 1e-9);
+            g.res[l] = (float)r;
+            g.side[l] = (uint32_t)r + 1;
+            g.dense[l] = ((double)g.side[l] * g.side[l] * g.side[l] <= (double)T) ? 1u : 0u;
+        }
+        for (int k = 0; k < 3; ++k) {
+            const float lo = scene->bbox[k], hi = scene->bbox[3 + k];
+            const float ext = std::max(hi - lo, 1e-6f);
+            g.bmin[k] = lo - 1e-3f * ext;
+            g.binv[k] = 1.0f / (ext * 1.002f);
+        }
+        g.grid_params = (size_t)kNrcLevels * T * 2;
+        NrcTrain &t = c->tr;
+        t.w = (float *)alloc(c, kNrcParams * sizeof(float));
+        t.m = (float *)alloc(c, kNrcParams * sizeof(float));
+        t.v = (float *)alloc(c, kNrcParams * sizeof(float));
+        t.w16 = (_Float16 *)alloc(c, kNrcParams * sizeof(_Float16));
+        t.wt16 = (_Float16 *)alloc(c, kNrcParams * sizeof(_Float16));
+        t.grid = (float *)alloc(c, g.grid_params * sizeof(float));
+        t.grid_m = (float *)alloc(c, g.grid_params * sizeof(float));
+        t.grid_v = (float *)alloc(c, g.grid_params * sizeof(float));
+        g.t16 = (_Float16 *)alloc(c, g.grid_params * sizeof(_Float16));
+        g.grad = (float *)alloc(c, g.grid_params * sizeof(float));
+        c->count = (uint32_t *)alloc(c, 256);
+        // initial parameters: Xavier-uniform MLP, small uniform grid
+        std::mt19937_64 rng(prm->seed ^ 0x5EEDCAC7ull);
+        std::vector<float> w(kNrcParams, 0.0f);
+        const float a = std::sqrt(6.0f / 128.0f);
+        std::uniform_real_distribution<float> U(-a, a);
+        for (int j = 0; j < 6; ++j) {
+            const int rows = j < 5 ? 64 : 3;
+            for (int r = 0; r < rows; ++r)
+                for (int col = 0; col < 64; ++col) w[j * 4096 + r * 64 + col] = U(rng);
+        }
+        std::vector<float> gw(g.grid_params);
+        std::uniform_real_distribution<float> G(-1e-4f, 1e-4f);
+        for (auto &x : gw) x = G(rng);
+        hipStream_t st = scene->stream;
+        hipchk(hipMemcpyAsync(t.w, w.data(), w.size() * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        hipchk(hipMemcpyAsync(t.grid, gw.data(), gw.size() * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        hipchk(nrc_launch_zero(t.m, kNrcParams, st), "zero");
+        hipchk(nrc_launch_zero(t.v, kNrcParams, st), "zero");
+        hipchk(nrc_launch_zero(t.grid_m, g.grid_params, st), "zero");
+        hipchk(nrc_launch_zero(t.grid_v, g.grid_params, st), "zero");
+        hipchk(nrc_launch_zero(g.grad, g.grid_params, st), "zero");
+        hipchk(nrc_launch_sync_weights(g, t, st), "sync");
+        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        ensure_batch(c, std::max<uint32_t>(prm->train_batch, 64));
+        return (int)RT_OK;
+    });
+    if (rc != RT_OK) {
+        delete c;
+        return rc;
+    }
+    scene->caches.push_back(c);
+    *out = c;
+    return RT_OK;
+}
+
+int rt_cache_destroy(rt_cache *c) {
+    if (c) {
+        if (c->scene) {
+            (void)hipStreamSynchronize(c->scene->stream);
+            auto &v = c->scene->caches;
+            v.erase(std::remove(v.begin(), v.end(), c), v.end());
+        }
+        (void)hipSetDevice(c->device);
+        delete c;
+    }
+    return RT_OK;
+}
+
+int rt_cache_layout(const rt_cache *c, float *res, uint32_t *dense, float *bmin, float *binv) {
+    if (!c) {
+        set_error("cache is NULL");
+        return RT_ERR_INVALID;
+    }
+    for (int l = 0; l < kNrcLevels; ++l) {
+        if (res) res[l] = c->grid.res[l];
+        if (dense) dense[l] = c->grid.dense[l];
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (bmin) bmin[k] = c->grid.bmin[k];
+        if (binv) binv[k] = c->grid.binv[k];
+    }
+    return RT_OK;
+}
+
+int rt_cache_get_params(rt_cache *c, float *mlp, float *grid) {
+    return guarded([&]() {
+        hipStream_t st = live_scene(c)->stream;
+        if (mlp) hipchk(hipMemcpyAsync(mlp, c->tr.w, kNrcParams * 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        if (grid)
+            hipchk(hipMemcpyAsync(grid, c->tr.grid, c->grid.grid_params * 4, hipMemcpyDeviceToHost, st),
+                   "hipMemcpyAsync");
+        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return (int)RT_OK;
+    });
+}
+
+int rt_cache_set_params(rt_cache *c, const float *mlp, const float *grid) {
+    return guarded([&]() {
+        hipStream_t st = live_scene(c)->stream;
+        if (mlp) {
+            std::vector<float> w(mlp, mlp + kNrcParams);
+            for (int r = 3; r < 16; ++r)
+                for (int col = 0; col < 64; ++col) w[5 * 4096 + r * 64 + col] = 0.0f;
+            hipchk(hipMemcpyAsync(c->tr.w, w.data(), kNrcParams * 4, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+            hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        }
+        if (grid)
+            hipchk(hipMemcpyAsync(c->tr.grid, grid, c->grid.grid_params * 4, hipMemcpyHostToDevice, st),
+                   "hipMemcpyAsync");
+        hipchk(nrc_launch_sync_weights(c->grid, c->tr, st), "sync");
+        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return (int)RT_OK;
+    });
+}
+
+int rt_cache_eval(rt_cache *c, uint32_t n, const float *queries, float *out) {
+    return guarded([&]() {
+        if (!c || (n && (!queries || !out))) throw std::invalid_argument("cache, queries and out are required");
+        hipStream_t st = live_scene(c)->stream;
+        if (n == 0) return (int)RT_OK;
+        ensure_batch(c, n);
+        hipchk(hipMemcpyAsync(c->samples, queries, (size_t)n * kNrcFloats * 4, hipMemcpyHostToDevice, st),
+               "hipMemcpyAsync");
+        hipchk(nrc_launch_infer(c->grid, c->tr.w16, c->samples, nullptr, n, (float4 *)c->tr.pred_out, nullptr,
+                                nullptr, (uint32_t)c->scene->cu_count * 4, 0, st),
+               "nrc infer");
+        hipchk(hipMemcpyAsync(out, c->tr.pred_out, (size_t)n * 16, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return (int)RT_OK;
+    });
+}
+
+int rt_cache_grad(rt_cache *c, uint32_t n, const float *samples, float *loss, float *grad_mlp, float *grad_grid) {
+    return guarded([&]() {
+        if (!c || !samples || n == 0) throw std::invalid_argument("cache and samples are required");
+        hipStream_t st = live_scene(c)->stream;
+        ensure_batch(c, n);
+        hipchk(hipMemcpyAsync(c->samples, samples, (size_t)n * kNrcFloats * 4, hipMemcpyHostToDevice, st),
+               "hipMemcpyAsync");
+        hipchk(nrc_launch_zero(c->grid.grad, c->grid.grid_params, st), "zero");
+        hipchk(nrc_launch_train_step(c->grid, c->tr, c->samples, n, st), "nrc train step");
+        const float l = step_loss(c, n);
+        if (loss) *loss = l;
+        const uint32_t waves = (n + 63) / 64;
+        const double inv = 1.0 / (3.0 * n);
+        if (grad_mlp) {
+            std::vector<float> part((size_t)waves * kNrcParams);
+            hipchk(hipMemcpy(part.data(), c->tr.wpart, part.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+            for (int p = 0; p < kNrcParams; ++p) {
+                double sacc = 0.0;
+                for (uint32_t w = 0; w < waves; ++w) sacc += part[(size_t)w * kNrcParams + p];
+                grad_mlp[p] = (float)(sacc * inv);
+            }
+            for (int r = 3; r < 16; ++r)
+                for (int col = 0; col < 64; ++col) grad_mlp[5 * 4096 + r * 64 + col] = 0.0f;
+        }
+        if (grad_grid) {
+            hipchk(hipMemcpy(grad_grid, c->grid.grad, c->grid.grid_params * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+            for (size_t i = 0; i < c->grid.grid_params; ++i) grad_grid[i] = (float)(grad_grid[i] * inv);
+        }
+        hipchk(nrc_launch_zero(c->grid.grad, c->grid.grid_params, st), "zero");
+        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return (int)RT_OK;
+    });
+}
+
+int rt_cache_train_samples(rt_cache *c, uint32_t n, const float *samples, uint32_t steps, float *loss_out) {
+    return guarded([&]() {
+        if (!c || !samples || n == 0) throw std::invalid_argument("cache and samples are required");
+        hipStream_t st = live_scene(c)->stream;
+        ensure_batch(c, n);
+        hipchk(hipMemcpyAsync(c->samples, samples, (size_t)n * kNrcFloats * 4, hipMemcpyHostToDevice, st),
+               "hipMemcpyAsync");
+        train_steps(c, c->samples, n, steps, loss_out);
+        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return (int)RT_OK;
+    });
+}
+
+int rt_cache_train(rt_cache *c, const rt_camera *cam, uint32_t max_depth, uint32_t steps, float *loss_out) {
+    return guarded([&]() {
+        if (!c || !cam) throw std::invalid_argument("cache and camera are required");
+        live_scene(c);
+        if (max_depth == 0 || max_depth > (uint32_t)kNrcMaxV) throw std::invalid_argument("max_depth must be in [1, 16]");
+        rt_scene *s = c->scene;
+        hipStream_t st = s->stream;
+        hipchk(hipSetDevice(s->device), "hipSetDevice");
+        const uint32_t batch = c->prm.train_batch;
+        const uint32_t cap = std::max(batch, c->prm.train_paths);
+        ensure_batch(c, cap);
+        NrcGen gp;
+        std::memset(&gp, 0, sizeof(gp));
+        gp.cam = *cam;
+        gp.divW = 1.0f / float(cam->width);
+        gp.divH = 1.0f / float(cam->height);
+        gp.npix = cam->width * cam->height;
+        gp.npaths = c->prm.train_paths;
+        gp.frame = c->frame++;
+        gp.max_depth = max_depth;
+        gp.first_vertex = c->prm.first_vertex;
+        gp.capacity = cap;
+        gp.key0 = (uint32_t)c->prm.seed;
+        gp.key1 = (uint32_t)(c->prm.seed >> 32) ^ 0x9E3779B9u;
+        gp.target_clamp = c->prm.target_clamp > 0.0f ? c->prm.target_clamp : 1e30f;
+        const uint32_t grid = persistent_grid(s, nrc_gen_blocks_per_cu());
+        ensure_overflow(s, grid * 256);
+        hipchk(nrc_launch_zero((float *)c->count, 1, st), "zero");
+        hipchk(nrc_launch_gen(s->ds, gp, c->samples, c->count, grid, s->d_ovf_ref, s->d_ovf_t, (int64_t)grid * 256,
+                              st),
+               "nrc gen");
+        uint32_t made = 0;
+        hipchk(hipMemcpyAsync(&made, c->count, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        const uint32_t n = std::min(made, std::min(batch, cap));
+        if (n == 0) {
+            if (loss_out) *loss_out = 0.0f;
+            return (int)RT_OK;
+        }
+        train_steps(c, c->samples, n, steps, loss_out);
+        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        return (int)RT_OK;
+    });
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@
 // randomness is Philox4x32-10 on (pixel, sample, depth, k), so the image does
 // not depend on batch size, queue order or launch shape.
 #include "dev_bsdf.h"
+#include "nrc_kernels.h"
 #include "pt_kernels.h"
 
 namespace rtamd {
@@ -23,6 +24,10 @@ namespace rtamd {
 // Per-batch counter block: for depth d, 4 counters (queue size, extend fetch,
 // shadow queue size, shadow fetch), each on its own 128-byte line.
 __device__ __forceinline__ uint32_t *ctr_at(const PtDev &p, uint32_t batch, uint32_t d, uint32_t k) {
+    return p.ctr + (((size_t)batch * kMaxDepth + d) * 4u + k) * 32u;
+}
+
+static uint32_t *ctr_at_host(const PtDev &p, uint32_t batch, uint32_t d, uint32_t k) {
     return p.ctr + (((size_t)batch * kMaxDepth + d) * 4u + k) * 32u;
 }
 
@@ -74,7 +79,8 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
                                                        uint32_t npaths, uint32_t depth) {
     const uint32_t count = FIRST ? npaths : *ctr_at(p, batch, depth, 0);
     const uint32_t stride = gridDim.x * blockDim.x;
-    const bool last = depth + 1 >= p.max_depth;
+    const bool term = p.cache_depth != 0 && depth == p.cache_depth;
+    const bool last = term || depth + 1 >= p.max_depth;
     uint32_t *q_out = (depth & 1u) ? p.q0 : p.q1;
     const uint32_t *q_in = (depth & 1u) ? p.q1 : p.q0;
     uint32_t *next_ctr = last ? nullptr : ctr_at(p, batch, depth + 1, 0);
@@ -83,7 +89,8 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t j = base + threadIdx.x;
         const bool active = j < count;
-        bool push_next = false, push_shadow = false;
+        bool push_next = false, push_shadow = false, push_query = false;
+        float qf[13];
         uint32_t slot = 0;
         if (active) {
             slot = FIRST ? j : q_in[j];
@@ -136,10 +143,20 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
                 const float pdf_e = emission_pdf(pos, o, neg(d), N, area);
                 V3 e_term = depth == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
                 L = add(L, mulv(T, e_term));
+                if (term) {
+                    // the cache stands for everything after this vertex's emission term
+                    push_query = true;
+                    qf[0] = pos.x; qf[1] = pos.y; qf[2] = pos.z;
+                    qf[3] = -d.x; qf[4] = -d.y; qf[5] = -d.z;
+                    qf[6] = ff.x; qf[7] = ff.y; qf[8] = ff.z;
+                    qf[9] = m.roughness;
+                    qf[10] = m.base.x; qf[11] = m.base.y; qf[12] = m.base.z;
+                    p.thr[slot] = make_float4(T.x, T.y, T.z, 0.0f);
+                }
                 const float len = length(sub(pos, o));
                 const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
                 // DirectLight (src/renderer.cpp:335-384)
-                if (sc.num_emit > 0) {
+                if (sc.num_emit > 0 && !term) {
                     const EmitterRec &E = sc.emit[pick(ra.x, sc.num_emit)];
                     const float sqr1 = sqrtf(u01(ra.y));
                     const float r2 = u01(ra.z);
@@ -200,6 +217,16 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
         if (push_next && next_ctr) q_out[qi] = slot;
         const uint32_t si = wave_append(sh_ctr, push_shadow);
         if (push_shadow) p.sq[si] = slot;
+        if (p.cache_depth) {
+            const uint32_t ci = wave_append(ctr_at(p, batch, depth, 3), push_query);
+            if (push_query) {
+                float *dst = p.cache_q + kNrcFloats * (size_t)ci;
+                for (int c = 0; c < 13; ++c) dst[c] = qf[c];
+                dst[kRecBounces] = (float)(p.max_depth - 1u - depth);
+                dst[kRecBounces + 1] = 0.0f;
+                dst[kRecSlot] = __uint_as_float(slot);
+            }
+        }
     }
 }
 
@@ -298,9 +325,10 @@ hipError_t pt_enqueue_primary(const DevScene &sc, const PtDev &p, uint32_t grid,
 
 hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t s0, uint32_t K,
                             uint32_t trace_grid, uint32_t shade_grid, uint32_t *ovf_ref, float *ovf_t,
-                            int64_t ovf_stride, hipStream_t s) {
+                            int64_t ovf_stride, const PtCache &cache, hipStream_t s) {
     const uint32_t npaths = K * p.npix;
-    for (uint32_t d = 0; d < p.max_depth; ++d) {
+    const uint32_t depth_end = p.cache_depth ? p.cache_depth + 1 : p.max_depth;
+    for (uint32_t d = 0; d < depth_end; ++d) {
         if (d == 0) {
             hipLaunchKernelGGL(pt_shade_kernel<true>, dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0, npaths,
                                0u);
@@ -309,6 +337,13 @@ hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, 
                                ovf_t, ovf_stride);
             hipLaunchKernelGGL(pt_shade_kernel<false>, dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0, npaths,
                                d);
+        }
+        if (p.cache_depth && d == p.cache_depth) {
+            // the queries of this depth were counted in its shadow-fetch counter (no shadow rays here)
+            hipError_t e = nrc_launch_infer(*cache.grid, cache.w16, p.cache_q, ctr_at_host(p, batch, d, 3), 0, nullptr,
+                                            p.rad, p.thr, cache.infer_grid, 1, s);
+            if (e != hipSuccess) return e;
+            break;
         }
         if (sc.num_emit > 0)
             hipLaunchKernelGGL(pt_trace_kernel<true>, dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d, ovf_ref,

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "nrc.h"
 #include "pt.h"
 #include "pt_kernels.h"
 #include "scene.h"
@@ -30,6 +31,8 @@ struct PtState {
     hipGraph_t graph = nullptr;
     std::vector<uint8_t> key;
     uint32_t batches = 0, K = 0;
+    float *cache_q = nullptr;       // radiance-cache queries (cap x 16 floats)
+    size_t cache_q_cap = 0;
 };
 
 namespace {
@@ -57,6 +60,8 @@ void free_all(PtState *st) {
     st->exec = nullptr;
     st->graph = nullptr;
     st->key.clear();
+    st->cache_q = nullptr;
+    st->cache_q_cap = 0;
 }
 
 }  // namespace
@@ -81,7 +86,8 @@ int pt_device_output(rt_scene *s, void **dev_ptr, size_t *bytes) {
 
 using namespace rtamd;
 
-extern "C" int rt_render_pt(rt_scene *s, const rt_camera *cam, const rt_pt_params *prm, float *rgb_host) {
+static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params *prm, float *rgb_host,
+                          rt_cache *cache) {
     try {
         if (!s || !cam || !prm) {
             set_error("scene, camera or params is NULL");
@@ -135,8 +141,16 @@ extern "C" int rt_render_pt(rt_scene *s, const rt_camera *cam, const rt_pt_param
             st->npix = npix;
             st->ctr_words = ctr_words;
         }
+        if (cache && st->cache_q_cap < cap) {
+            st->cache_q = (float *)alloc(st, cap * kNrcFloats * sizeof(float));
+            st->cache_q_cap = cap;
+        }
         PtDev &p = st->dev;
         p.cam = *cam;
+        p.cache_depth = cache ? cache->prm.first_vertex : 0;
+        p.cache_q = cache ? st->cache_q : nullptr;
+        PtCache pc{cache ? &cache->grid : nullptr, cache ? cache->tr.w16 : nullptr,
+                   (uint32_t)s->cu_count * 4};
         p.divW = 1.0f / float(cam->width);
         p.divH = 1.0f / float(cam->height);
         p.npix = npix;
@@ -157,16 +171,19 @@ extern "C" int rt_render_pt(rt_scene *s, const rt_camera *cam, const rt_pt_param
                 uint32_t s0 = b * K;
                 uint32_t Kb = std::min(K, prm->spp - s0);
                 hipchk(pt_enqueue_batch(s->ds, p, b, s0, Kb, trace_grid, shade_grid, s->d_ovf_ref, s->d_ovf_t,
-                                        ovf_stride, strm),
+                                        ovf_stride, pc, strm),
                        "batch");
             }
         };
 
         // graph key: everything the captured launches depend on
-        std::vector<uint8_t> key(sizeof(PtDev) + 64);
+        // (a cache enters through its buffers, which a new cache at a reused address does not share)
+        uint64_t extra[10] = {nb, K, trace_grid, shade_grid, (uint64_t)(uintptr_t)s->d_ovf_ref,
+                              (uint64_t)(uintptr_t)s->d_ovf_t, (uint64_t)ovf_stride, ctr_words,
+                              (uint64_t)(uintptr_t)(cache ? cache->grid.t16 : nullptr),
+                              (uint64_t)(uintptr_t)(cache ? cache->tr.w16 : nullptr)};
+        std::vector<uint8_t> key(sizeof(PtDev) + sizeof(extra));
         std::memcpy(key.data(), &p, sizeof(PtDev));
-        uint64_t extra[8] = {nb, K, trace_grid, shade_grid, (uint64_t)(uintptr_t)s->d_ovf_ref,
-                             (uint64_t)(uintptr_t)s->d_ovf_t, (uint64_t)ovf_stride, ctr_words};
         std::memcpy(key.data() + sizeof(PtDev), extra, sizeof(extra));
         const bool use_graph = !(prm->flags & RT_PT_NO_GRAPH);
 
@@ -230,4 +247,18 @@ extern "C" int rt_render_pt(rt_scene *s, const rt_camera *cam, const rt_pt_param
         set_error(e.what());
         return RT_ERR_HIP;
     }
+}
+
+extern "C" int rt_render_pt(rt_scene *s, const rt_camera *cam, const rt_pt_params *prm, float *rgb_host) {
+    return render_pt_impl(s, cam, prm, rgb_host, nullptr);
+}
+
+extern "C" int rt_render_pt_cached(rt_scene *s, rt_cache *cache, const rt_camera *cam, const rt_pt_params *prm,
+                                   float *rgb_host) {
+    if (!cache || cache->scene != s) {
+        set_error("cache is NULL or belongs to another scene");
+        return RT_ERR_INVALID;
+    }
+    if (cache->prm.first_vertex >= prm->max_depth) return render_pt_impl(s, cam, prm, rgb_host, nullptr);
+    return render_pt_impl(s, cam, prm, rgb_host, cache);
 }

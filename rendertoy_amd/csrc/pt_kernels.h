@@ -29,14 +29,24 @@ struct PtDev {
     uint32_t *ctr;        // kMaxDepth * 4 counters per batch, 32-word stride
     uint32_t *prim_ctr;   // 8 band counters, 32-word stride
     float *fb;            // npix * 3
+    // radiance cache: paths reaching vertex cache_depth (>= 1; 0 = off) add their
+    // emission term there and leave a cache query (16 floats, slot in [13]).
+    uint32_t cache_depth;
+    float *cache_q;
 };
 
 int pt_trace_blocks_per_cu();
 hipError_t pt_enqueue_frame_start(const PtDev &p, size_t ctr_words, hipStream_t s);
 hipError_t pt_enqueue_primary(const DevScene &sc, const PtDev &p, uint32_t grid, uint32_t *ovf_ref, float *ovf_t,
                               int64_t ovf_stride, hipStream_t s);
+struct NrcGrid;
+struct PtCache {
+    const NrcGrid *grid;        // null: plain path tracing
+    const _Float16 *w16;
+    uint32_t infer_grid;
+};
 hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t s0, uint32_t K,
                             uint32_t trace_grid, uint32_t shade_grid, uint32_t *ovf_ref, float *ovf_t,
-                            int64_t ovf_stride, hipStream_t s);
+                            int64_t ovf_stride, const PtCache &cache, hipStream_t s);
 
 }  // namespace rtamd

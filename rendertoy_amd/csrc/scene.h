@@ -32,6 +32,7 @@ struct rt_scene {
     uint32_t num_tris = 0;
     uint32_t bvh_nodes = 0, bvh_depth = 0, max_stack = 0;
     double build_ms = 0.0;
+    float bbox[6] = {0, 0, 0, 0, 0, 0};   // scene bounds (min xyz, max xyz)
     bool counting = false;
     // work queue counters (8 x 128-byte lines) and statistics
     uint32_t *d_counters = nullptr;
@@ -50,6 +51,13 @@ struct rt_scene {
     // path tracer state
     rtamd::PtState *pt = nullptr;
     rt_stats stats{};
+    // radiance caches built on this scene (detached when the scene goes first)
+    std::vector<rt_cache *> caches;
 
     ~rt_scene();
 };
+
+namespace rtamd {
+// Called by rt_scene_destroy: the scene's radiance caches stop referring to it.
+void detach_caches(rt_scene *scene);
+}  // namespace rtamd

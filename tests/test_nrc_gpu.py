@@ -1,0 +1,253 @@
+"""Radiance cache on the GPU (rendertoy_amd/csrc/nrc.hip) against the oracle.
+
+The oracle (oracle/nrc_oracle.py, pinned in test_nrc_oracle.py) applies the
+device's fp16 casts; the device sums in a different order, so an fp16
+rounding can land one ulp apart and move the output slightly.  Tolerances:
+predictions within 1% of the output scale (max) and 0.1% (mean); gradients
+within 2% in relative L2 norm per weight matrix and over the grid.
+
+The render hook is checked through exact properties of the shared random
+stream: with a cache that predicts zero, every path ends at vertex v after
+its emission term, so per pixel plain(max_depth=v) <= cached <= plain(v+1);
+predictions are linear in the output matrix, so doubling it doubles the
+cache's share of the image.
+"""
+import numpy as np
+import pytest
+
+from helpers import scene
+from oracle import nrc_oracle as N
+
+pytestmark = pytest.mark.gpu
+
+LOG2 = 14
+
+
+def make_cache(s, **kw):
+    from rendertoy_amd import capi
+    prm = dict(log2_table=LOG2, train_batch=4096, train_paths=4096)
+    prm.update(kw)
+    return capi.Cache(s, **prm)
+
+
+def oracle_layout(c):
+    from oracle import nrc_oracle as N
+    L = c.layout()
+    return N.Layout(L["res"], L["dense"], L["bmin"], L["binv"], L["log2_table"])
+
+
+def records(sc, rng, n):
+    v = sc.verts.reshape(-1, 3)
+    lo, hi = v.min(0), v.max(0)
+    q = np.zeros((n, N.RECORD), np.float32)
+    q[:, 0:3] = lo + (hi - lo) * rng.uniform(0, 1, (n, 3))
+    d = rng.normal(size=(n, 3))
+    q[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    nn = rng.normal(size=(n, 3))
+    q[:, 6:9] = nn / np.linalg.norm(nn, axis=1, keepdims=True)
+    q[:, 9] = rng.uniform(0, 1, n)
+    q[:, 10:13] = rng.uniform(0, 1, (n, 3))
+    q[:, N.BOUNCES] = rng.integers(0, 8, n)
+    q[:, N.TARGET:N.TARGET + 3] = rng.uniform(0, 2, (n, 3))
+    return q
+
+
+def random_params(rng, c):
+    from oracle import nrc_oracle as N
+    a = np.sqrt(6.0 / 128.0)
+    mlp = rng.uniform(-a, a, N.N_PARAMS).astype(np.float32)
+    mlp[5 * 4096 + 3 * 64:] = 0.0
+    grid = rng.uniform(-0.5, 0.5, (N.LEVELS, c.table, 2)).astype(np.float32)
+    return mlp, grid
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    from rendertoy_amd import capi
+    sc = scene("cornell")
+    return sc, capi.Scene(sc)
+
+
+def test_cache_layout_matches_oracle(cornell):
+    from oracle import nrc_oracle as N
+    sc, s = cornell
+    c = make_cache(s, log2_table=19)
+    L = c.layout()
+    v = sc.verts.reshape(-1, 3)
+    ref = N.Layout.make(19, 16, 2048, list(v.min(0)) + list(v.max(0)))
+    assert np.array_equal(L["res"], ref.res)
+    assert np.array_equal(L["dense"], ref.dense)
+    assert np.array_equal(L["bmin"], ref.bmin)
+    assert np.array_equal(L["binv"], ref.binv)
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 4160])
+def test_cache_eval_matches_oracle(cornell, n):
+    from oracle import nrc_oracle as N
+    sc, s = cornell
+    print("case eval", n, flush=True)
+    rng = np.random.default_rng(n)
+    c = make_cache(s)
+    mlp, grid = random_params(rng, c)
+    c.set_params(mlp, grid)
+    m2, g2 = c.get_params()
+    assert np.array_equal(m2, mlp) and np.array_equal(g2, grid)
+    q = records(sc, rng, n)
+    g = c.eval(q)
+    o, _, _ = N.forward(oracle_layout(c), mlp, grid, q, fp16=True)
+    scale = max(1.0, float(np.abs(o).max()))
+    err = np.abs(g - o)
+    print("max err", err.max(), "mean err", err.mean(), "scale", scale, flush=True)
+    assert err.max() <= 1e-2 * scale
+    assert err.mean() <= 1e-3 * scale
+
+
+def test_cache_grad_matches_oracle(cornell):
+    from oracle import nrc_oracle as N
+    sc, s = cornell
+    rng = np.random.default_rng(11)
+    c = make_cache(s)
+    mlp, grid = random_params(rng, c)
+    c.set_params(mlp, grid)
+    q = records(sc, rng, 333)
+    loss, gm, gg = c.grad(q)
+    oloss, ogm, ogg, _ = N.loss_and_grads(oracle_layout(c), mlp, grid, q, fp16=True)
+    print("loss", loss, oloss, flush=True)
+    assert abs(loss - oloss) <= 1e-2 * abs(oloss)
+    for j in range(6):
+        sl = slice(j * 4096, (j + 1) * 4096) if j < 5 else slice(5 * 4096, 5 * 4096 + 3 * 64)
+        rel = np.linalg.norm(gm[sl] - ogm[sl]) / np.linalg.norm(ogm[sl])
+        print("W%d rel" % j, rel, flush=True)
+        assert rel <= 2e-2
+    assert np.all(gm[5 * 4096 + 3 * 64:] == 0)
+    rel = np.linalg.norm(gg - ogg) / np.linalg.norm(ogg)
+    print("grid rel", rel, flush=True)
+    assert rel <= 2e-2
+    # the touched entries and only those carry gradient
+    assert np.count_nonzero(gg) <= np.count_nonzero(ogg) + 16
+
+
+def test_cache_adam_first_step(cornell):
+    """A first Adam step moves each parameter by about lr against its gradient."""
+    from oracle import nrc_oracle as N
+    sc, s = cornell
+    rng = np.random.default_rng(5)
+    c = make_cache(s, lr_mlp=1e-3, lr_grid=1e-2)
+    mlp, grid = random_params(rng, c)
+    c.set_params(mlp, grid)
+    q = records(sc, rng, 512)
+    _, gm, gg = c.grad(q)
+    c.train_samples(q, steps=1)
+    m1, g1 = c.get_params()
+    # (eps = 1e-10 still bends the step where |g| is near 1e-8)
+    want, _, _ = N.adam(mlp.astype(np.float64), 0.0, 0.0, gm.astype(np.float64), 1, 1e-3)
+    big = np.abs(gm) > 1e-6 * np.abs(gm).max()
+    np.testing.assert_allclose(m1[big], want[big], rtol=0, atol=2e-6)
+    assert np.array_equal(m1[gm == 0], mlp[gm == 0])
+    want, _, _ = N.adam(grid.astype(np.float64), 0.0, 0.0, gg.astype(np.float64), 1, 1e-2)
+    bigg = np.abs(gg) > 1e-6 * np.abs(gg).max()
+    np.testing.assert_allclose(g1[bigg], want[bigg], rtol=0, atol=2e-6)
+    assert np.array_equal(g1[gg == 0], grid[gg == 0])
+
+
+def test_cache_learns_a_smooth_field(cornell):
+    """Targets that are a smooth function of position and direction are
+    learned: the relative L2 loss falls at least tenfold."""
+    sc, s = cornell
+    rng = np.random.default_rng(9)
+    c = make_cache(s, train_batch=16384, lr_mlp=1e-2, lr_grid=1e-2)
+    q = records(sc, rng, 16384)
+    v = sc.verts.reshape(-1, 3)
+    u = (q[:, 0:3] - v.min(0)) / (v.max(0) - v.min(0))
+    T = N.TARGET
+    q[:, T] = 0.5 + 0.4 * np.sin(6 * u[:, 0]) * np.cos(4 * u[:, 1])
+    q[:, T + 1] = 0.3 + 0.25 * q[:, 5] + 0.2 * u[:, 2]
+    q[:, T + 2] = 0.6 * u[:, 0] * u[:, 1] + 0.1
+    first = c.train_samples(q, steps=1)
+    last = c.train_samples(q, steps=400)
+    print("loss", first, "->", last, flush=True)
+    assert np.isfinite(last) and last < 0.1 * first
+    pred = c.eval(q)
+    assert np.abs(pred - q[:, T:T + 3]).mean() < 0.05
+
+
+def test_cache_train_from_paths(cornell):
+    sc, s = cornell
+    c = make_cache(s, train_batch=16384, train_paths=16384)
+    losses = [c.train(steps=8) for _ in range(12)]
+    print("losses", losses, flush=True)
+    assert all(np.isfinite(losses)) and all(l >= 0 for l in losses)
+    assert np.mean(losses[-3:]) < np.mean(losses[:3])
+
+
+def test_cached_render_sandwich(cornell):
+    """Zero cache: plain(max_depth=v) <= cached <= plain(max_depth=v+1)."""
+    sc, s = cornell
+    c = make_cache(s, first_vertex=2)
+    mlp, _ = c.get_params(grid=False)
+    c.set_params(np.zeros_like(mlp))
+    z = c.render_pt(spp=8, max_depth=5, seed=3)
+    lo = s.render_pt(spp=8, max_depth=2, seed=3)
+    hi = s.render_pt(spp=8, max_depth=3, seed=3)
+    ok = np.isfinite(z) & np.isfinite(lo) & np.isfinite(hi)    # (coplanar light samples give NaN in RenderToy)
+    assert ok.mean() > 0.999
+    z, lo, hi = z[ok], lo[ok], hi[ok]
+    tol = 1e-6 * np.maximum(1.0, hi)
+    assert np.all(z >= lo - tol) and np.all(z <= hi + tol)
+    assert z.mean() > lo.mean() * (1 + 1e-3) and z.mean() < hi.mean() * (1 - 1e-3)
+
+
+def test_cached_render_linear_in_output_layer(cornell):
+    sc, s = cornell
+    rng = np.random.default_rng(2)
+    c = make_cache(s, first_vertex=1)
+    mlp, grid = random_params(rng, c)
+    mlp[5 * 4096:5 * 4096 + 3 * 64] = np.abs(mlp[5 * 4096:5 * 4096 + 3 * 64])   # predictions >= 0
+    zero = mlp.copy()
+    zero[5 * 4096:] = 0.0
+    c.set_params(zero, grid)
+    z = c.render_pt(spp=4, max_depth=5, seed=1)
+    c.set_params(mlp, grid)
+    x1 = c.render_pt(spp=4, max_depth=5, seed=1)
+    x1b = c.render_pt(spp=4, max_depth=5, seed=1)
+    two = mlp.copy()
+    two[5 * 4096:] *= 2.0
+    c.set_params(two, grid)
+    x2 = c.render_pt(spp=4, max_depth=5, seed=1)
+    assert np.array_equal(x1, x1b)          # graph replay, same weights: same frame
+    d1, d2 = x1 - z, x2 - z
+    assert d1.mean() > 0
+    np.testing.assert_allclose(d2, 2 * d1, rtol=1e-4, atol=1e-5 * max(1.0, float(x2.max())))
+
+
+def test_cached_render_past_max_depth_is_plain(cornell):
+    sc, s = cornell
+    c = make_cache(s, first_vertex=5)
+    a = c.render_pt(spp=4, max_depth=5, seed=4)
+    b = s.render_pt(spp=4, max_depth=5, seed=4)
+    assert np.array_equal(a, b)
+
+
+def test_cache_rejects_bad_params(cornell):
+    from rendertoy_amd import capi
+    sc, s = cornell
+    with pytest.raises(capi.RtError):
+        make_cache(s, first_vertex=0)
+    with pytest.raises(capi.RtError):
+        make_cache(s, log2_table=30)
+    c = make_cache(s)
+    with pytest.raises(ValueError):
+        c.eval(np.zeros((4, 16), np.float32))
+
+
+def test_cache_outlives_scene():
+    """Destroying the scene first detaches the cache instead of leaving it dangling."""
+    from rendertoy_amd import capi
+    sc = scene("cornell")
+    s = capi.Scene(sc)
+    c = make_cache(s)
+    c.layout()
+    s.close()
+    with pytest.raises(capi.RtError):
+        c.eval(np.zeros((4, N.RECORD), np.float32))
+    c.close()

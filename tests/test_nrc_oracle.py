@@ -1,0 +1,169 @@
+"""The radiance-cache oracle, pinned on the CPU by what the mathematics fixes.
+
+- backward: float64 gradients against central finite differences of the loss
+  with its denominator p^2 + 0.01 frozen (the loss treats it as a constant);
+- encoding: trilinear interpolation reproduces a linear function of the grid
+  vertex coordinates exactly on a densely indexed level, and equals the table
+  entry at a grid vertex on a hashed level;
+- spherical harmonics: orthonormal over the sphere (Gauss-Legendre x uniform
+  azimuth quadrature, exact for these polynomial degrees);
+- Adam: the first step moves every parameter with a clear gradient by lr
+  against the gradient's sign.
+"""
+import numpy as np
+import pytest
+
+from oracle import nrc_oracle as N
+
+
+def small_layout(log2_table=10):
+    return N.Layout.make(log2_table, 2, 40, [0, 0, 0, 1, 1, 1])
+
+
+def random_records(rng, n):
+    q = np.zeros((n, N.RECORD), np.float32)
+    q[:, 0:3] = rng.uniform(0.02, 0.98, (n, 3))
+    d = rng.normal(size=(n, 3))
+    q[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    nn = rng.normal(size=(n, 3))
+    q[:, 6:9] = nn / np.linalg.norm(nn, axis=1, keepdims=True)
+    q[:, 9] = rng.uniform(0, 1, n)
+    q[:, 10:13] = rng.uniform(0, 1, (n, 3))
+    q[:, N.BOUNCES] = rng.integers(0, 8, n)
+    q[:, N.TARGET:N.TARGET + 3] = rng.uniform(0, 2, (n, 3))
+    return q
+
+
+def random_params(rng, lay, scale_grid=0.5):
+    mlp = rng.uniform(-0.3, 0.3, N.N_PARAMS)
+    mlp[5 * 4096 + 3 * 64:] = 0.0
+    grid = rng.uniform(-scale_grid, scale_grid, (N.LEVELS, lay.T, 2))
+    return mlp, grid
+
+
+def test_layout_levels():
+    lay = N.Layout.make(19, 16, 2048, [-1, -2, -3, 1, 2, 3])
+    assert lay.res[0] == 16 and lay.res[-1] == 2048
+    assert np.all(np.diff(lay.res) > 0)
+    # dense exactly while (R+1)^3 <= 2^19, i.e. R <= 79
+    assert np.array_equal(lay.dense.astype(bool), lay.res <= 79)
+    u = lay.unit(np.array([[-1, -2, -3], [1, 2, 3]], np.float32))
+    assert np.all(u[0] > 0) and np.all(u[0] < 1e-3) and np.all(u[1] < 1) and np.all(u[1] > 1 - 1e-3)
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(7)
+    lay = small_layout()
+    mlp, grid = random_params(rng, lay)
+    q = random_records(rng, 24)
+    loss, gm, gg, p = N.loss_and_grads(lay, mlp, grid, q, fp16=False)
+    den = p * p + 0.01
+    y = q[:, N.TARGET:N.TARGET + 3].astype(np.float64)
+
+    def L(m, g):
+        pp, _, _ = N.forward(lay, m, g, q, fp16=False)
+        return np.sum((pp - y) ** 2 / den) / (3 * len(q))
+
+    assert abs(L(mlp, grid) - loss) < 1e-12
+    eps = 1e-6
+    # MLP: a spread of entries of every matrix (W5 only over its 3 live rows)
+    picks = np.concatenate([j * 4096 + rng.choice(4096, 12, replace=False) for j in range(5)] +
+                           [5 * 4096 + rng.choice(3 * 64, 12, replace=False)])
+    for i in picks:
+        a, b = mlp.copy(), mlp.copy()
+        a[i] += eps
+        b[i] -= eps
+        fd = (L(a, grid) - L(b, grid)) / (2 * eps)
+        assert abs(fd - gm[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, gm[i])
+    # grid: entries the samples touch, on coarse and fine levels
+    pu = lay.unit(q[:, 0:3])
+    for l in (0, 5, 15):
+        idx, _ = lay.corners(pu, l)
+        for e in np.unique(idx[:3].ravel())[:6]:
+            for f in (0, 1):
+                a, b = grid.copy(), grid.copy()
+                a[l, e, f] += eps
+                b[l, e, f] -= eps
+                fd = (L(mlp, a) - L(mlp, b)) / (2 * eps)
+                assert abs(fd - gg[l, e, f]) <= 1e-7 + 1e-4 * abs(fd), (l, e, f, fd, gg[l, e, f])
+    # the padded output rows carry no gradient
+    assert np.all(gm[5 * 4096 + 3 * 64:] == 0)
+
+
+def test_fp16_path_close_to_float64():
+    rng = np.random.default_rng(3)
+    lay = small_layout()
+    mlp, grid = random_params(rng, lay)
+    q = random_records(rng, 256)
+    p16, _, _ = N.forward(lay, mlp, grid, q, fp16=True)
+    p64, _, _ = N.forward(lay, mlp, grid, q, fp16=False)
+    assert np.max(np.abs(p16 - p64)) < 2e-2 * max(1.0, np.max(np.abs(p64)))
+
+
+def test_trilinear_reproduces_linear_function_on_dense_level():
+    lay = small_layout(log2_table=14)
+    l = 1
+    assert lay.dense[l]
+    R = int(lay.res[l])
+    side = R + 1
+    grid = np.zeros((N.LEVELS, lay.T, 2), np.float64)
+    c = np.arange(side)
+    cz, cy, cx = np.meshgrid(c, c, c, indexing="ij")
+    idx = (cx + side * (cy + side * cz)).ravel()
+    grid[l, idx, 0] = (cx + 2 * cy + 3 * cz).ravel()
+    grid[l, idx, 1] = (5 - cz).ravel()
+    rng = np.random.default_rng(1)
+    q = random_records(rng, 200)
+    x = N.encode(lay, grid, q, fp16=False)
+    pu = lay.unit(q[:, 0:3]).astype(np.float64)
+    np.testing.assert_allclose(x[:, 2 * l], R * (pu[:, 0] + 2 * pu[:, 1] + 3 * pu[:, 2]), rtol=1e-5, atol=1e-4)
+    np.testing.assert_allclose(x[:, 2 * l + 1], 5 - R * pu[:, 2], rtol=1e-5, atol=1e-4)
+
+
+def test_hashed_level_vertex_value():
+    lay = small_layout(log2_table=8)
+    l = 15
+    assert not lay.dense[l]
+    R = lay.res[l]
+    rng = np.random.default_rng(2)
+    grid = rng.uniform(-1, 1, (N.LEVELS, lay.T, 2))
+    ijk = rng.integers(0, int(R), (50, 3)).astype(np.uint32)
+    # positions exactly on vertices (dyadic scene box so unit() is exact)
+    lay.bmin = np.zeros(3, np.float32)
+    lay.binv = np.ones(3, np.float32)
+    q = random_records(rng, 50)
+    q[:, 0:3] = ijk / np.float32(R)
+    pu = lay.unit(q[:, 0:3])
+    keep = np.all(pu * np.float32(R) == ijk, axis=1)    # on the vertex exactly
+    assert keep.sum() >= 20
+    q, ijk = q[keep], ijk[keep]
+    x = N.encode(lay, grid, q, fp16=False)
+    with np.errstate(over="ignore"):
+        h = (ijk[:, 0] ^ (ijk[:, 1] * np.uint32(2654435761)) ^ (ijk[:, 2] * np.uint32(805459861))) & np.uint32(lay.T - 1)
+    np.testing.assert_allclose(x[:, 2 * l], grid[l, h, 0], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(x[:, 2 * l + 1], grid[l, h, 1], rtol=1e-12, atol=1e-12)
+
+
+def test_sh_orthonormal():
+    ct, wt = np.polynomial.legendre.leggauss(12)
+    nphi = 16
+    phi = (np.arange(nphi) + 0.5) * 2 * np.pi / nphi
+    C, P = np.meshgrid(ct, phi, indexing="ij")
+    st = np.sqrt(1 - C * C)
+    d = np.stack([(st * np.cos(P)).ravel(), (st * np.sin(P)).ravel(), C.ravel()], 1).astype(np.float32)
+    w = (np.repeat(wt, nphi) * 2 * np.pi / nphi)
+    Y = N.sh16(d).astype(np.float64)
+    G = (Y * w[:, None]).T @ Y
+    np.testing.assert_allclose(G, np.eye(16), atol=2e-6)
+
+
+def test_adam_first_step_is_lr_times_sign():
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=100)
+    g = rng.normal(size=100)
+    nw, m, v = N.adam(w, np.zeros(100), np.zeros(100), g, 1, 0.01)
+    np.testing.assert_allclose(nw, w - 0.01 * np.sign(g), rtol=0, atol=1e-9)
+    # a constant gradient keeps the step at lr
+    for k in range(2, 6):
+        nw, m, v = N.adam(nw, m, v, g, k, 0.01)
+    np.testing.assert_allclose(nw, w - 0.05 * np.sign(g), rtol=0, atol=1e-8)

@@ -20,6 +20,10 @@ cpu_baseline: RenderToy itself (oracle/_ref, compiled from the reference's
 sources) on the host cores: BVH::Intersect on the exact AlbedoRenderer rays
 of a bounded pixel sample; falls back to the oracle restatement when the
 reference build is absent.
+
+--cache-study prints a different line instead: the radiance cache against
+plain path tracing on the terrain scene at equal time, RMSE against a plain
+65,536-spp frame accumulated in float64.
 """
 from __future__ import annotations
 
@@ -96,6 +100,115 @@ def cpu_baseline_c0(sc, spp):
                 sc.width, sc.height, spp)}
 
 
+def _rmse(img, ref, ok):
+    d = img.astype(np.float64)[ok] - ref[ok]
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def _timed(fn, reps=3):
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return out, float(np.median(ts)) * 1e3
+
+
+def _plain_rmse_at(ms, plain):
+    """Plain RMSE at a given frame time: log-log interpolation between the
+    measured (time, RMSE) points, extrapolated along RMSE ~ time^-1/2."""
+    ts = np.array([p["ms"] for p in plain])
+    rs = np.array([p["rmse"] for p in plain])
+    o = np.argsort(ts)
+    ts, rs = ts[o], rs[o]
+    if ms <= ts[0]:
+        return float(rs[0] * np.sqrt(ts[0] / ms))
+    if ms >= ts[-1]:
+        return float(rs[-1] * np.sqrt(ts[-1] / ms))
+    return float(np.exp(np.interp(np.log(ms), np.log(ts), np.log(rs))))
+
+
+def cache_study(args):
+    """Equal-time RMSE of cached vs plain path tracing on the terrain scene."""
+    from rendertoy_amd import capi, scenes
+    W, H, depth = args.cache_width, args.cache_height, 8
+    sc = scenes.terrain_c2(W, H)
+    s = capi.Scene(sc)
+    # plain reference: ref_spp samples per pixel in frames of 1024, summed in float64
+    fr = min(1024, args.ref_spp)
+    frames = max(1, args.ref_spp // fr)
+    acc = np.zeros((W * H, 3), np.float64)
+    cnt = np.zeros(W * H, np.int64)
+    t0 = time.perf_counter()
+    for i in range(frames):
+        img = s.render_pt(spp=fr, max_depth=depth, seed=1000 + i).astype(np.float64)
+        ok = np.isfinite(img).all(1)
+        acc[ok] += img[ok]
+        cnt += ok
+        if (i + 1) % 16 == 0:
+            log("reference frame %d/%d" % (i + 1, frames))
+    ref = acc / np.maximum(cnt, 1)[:, None]
+    ref_ok = cnt == frames
+    log("reference %d spp in %.1f s" % (fr * frames, time.perf_counter() - t0))
+    plain = []
+    for spp in (1, 2, 4, 8, 16, 32, 64, 128):
+        img, ms = _timed(lambda: s.render_pt(spp=spp, max_depth=depth, seed=7))
+        ok = ref_ok & np.isfinite(img).all(1)
+        plain.append({"spp": spp, "ms": round(ms, 3), "rmse": _rmse(img, ref, ok)})
+    c = capi.Cache(s, train_batch=65536, train_paths=65536, first_vertex=args.cache_vertex,
+                   lr_mlp=1e-3, lr_grid=1e-2, seed=3)
+    t0 = time.perf_counter()
+    losses = []
+    for i in range(args.cache_train):
+        losses.append(c.train(max_depth=depth, steps=4))
+    train_s = time.perf_counter() - t0
+    _, train_call_ms = _timed(lambda: c.train(max_depth=depth, steps=1))
+    cached = []
+    for spp in (1, 4, 16, 64):
+        img, ms = _timed(lambda: c.render_pt(spp=spp, max_depth=depth, seed=7))
+        q = s.stats()["cache_queries"]
+        ok = ref_ok & np.isfinite(img).all(1)
+        r = _rmse(img, ref, ok)
+        pr = _plain_rmse_at(ms, plain)
+        pr_online = _plain_rmse_at(ms + train_call_ms, plain)
+        cached.append({"spp": spp, "ms": round(ms, 3), "rmse": r, "cache_queries": int(q),
+                       "plain_rmse_same_time": pr, "rmse_ratio_plain_over_cached": round(pr / r, 3),
+                       "online_ms": round(ms + train_call_ms, 3),
+                       "online_rmse_ratio": round(pr_online / r, 3)})
+    img = c.render_pt(spp=1024, max_depth=depth, seed=8)
+    ok = ref_ok & np.isfinite(img).all(1)
+    bias = _rmse(img, ref, ok)
+    # full-resolution throughput, plain vs cached
+    scf = scenes.terrain_c2()
+    sf = capi.Scene(scf)
+    cf = capi.Cache(sf, train_batch=65536, train_paths=65536, first_vertex=args.cache_vertex, seed=3)
+    for _ in range(50):
+        cf.train(max_depth=depth, steps=4)
+    full = {}
+    for spp in (4, 16, 64):
+        _, pms = _timed(lambda: sf.render_pt(spp=spp, max_depth=depth, seed=1, download=False), reps=2)
+        _, cms = _timed(lambda: cf.render_pt(spp=spp, max_depth=depth, seed=1, download=False), reps=2)
+        n = scf.width * scf.height * spp
+        full["spp%d" % spp] = {"plain_msamples_per_s": round(n / pms / 1e3, 1),
+                               "cached_msamples_per_s": round(n / cms / 1e3, 1),
+                               "plain_ms": round(pms, 2), "cached_ms": round(cms, 2)}
+    best = max(cached, key=lambda e: e["rmse_ratio_plain_over_cached"])
+    out = {"metric": "equal-time RMSE vs 65,536-spp plain oracle (cached vs plain)",
+           "value": best["rmse_ratio_plain_over_cached"], "unit": "plain RMSE / cached RMSE at equal frame time",
+           "higher_is_better": True, "dtype": "f32 (cache MLP fp16 MFMA, fp32 accumulate)", "data": "synthetic",
+           "config": {"workload": "terrain (2,424,832 tris), %dx%d, depth 8, cache at vertex %d" % (
+               W, H, args.cache_vertex), "reference_spp": fr * frames, "reference_accumulation": "float64",
+               "reference_finite_pixels": float(ref_ok.mean()),
+               "cache": {"levels": 16, "table": 1 << 19, "mlp": "64-64x5-3 ReLU", "train_batch": 65536,
+                         "train_calls": args.cache_train, "steps_per_call": 4}},
+           "plain": plain, "cached": cached, "cached_1024spp_rmse": bias,
+           "train": {"seconds": round(train_s, 3), "call_ms_65536_paths_1_step": round(train_call_ms, 3),
+                     "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None},
+           "full_resolution_1920x1080": full}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +217,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true")
+    ap.add_argument("--cache-study", action="store_true",
+                    help="radiance cache vs plain path tracing at equal time (a different JSON line)")
+    ap.add_argument("--cache-width", type=int, default=480)
+    ap.add_argument("--cache-height", type=int, default=270)
+    ap.add_argument("--ref-spp", type=int, default=65536)
+    ap.add_argument("--cache-train", type=int, default=200, help="training calls of 4 Adam steps")
+    ap.add_argument("--cache-vertex", type=int, default=1)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (3 times) to calibrate HBM counters")
     args = ap.parse_args()
@@ -121,6 +241,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would render its own "
               "rows or tiles with no exchange until a final gather); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.cache_study:
+        cache_study(args)
+        return
     from rendertoy_amd import capi, scenes
 
     arch, cus = capi.device_info()

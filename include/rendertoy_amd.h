@@ -139,7 +139,9 @@ typedef struct rt_pt_params {
 
 enum rt_pt_flags {
     RT_PT_NO_GRAPH = 1u << 0,    /* launch the bounce loop kernel by kernel instead of replaying a hipGraph */
-    RT_PT_NO_SORT = 1u << 1      /* keep extension rays in path order (no coherence sort) */
+    RT_PT_NO_SORT = 1u << 1,     /* keep extension rays in path order (no coherence sort) */
+    RT_PT_NO_TUNE = 1u << 2      /* fixed trace-queue policy; by default the first renders of a new
+                                    frame shape try each policy once and keep the fastest */
 };
 
 /* PathTracingRenderer::Render (src/renderer.cpp:236-270) with Radiance and
@@ -222,6 +224,7 @@ typedef struct rt_stats {
     double build_ms;              /* host BVH build time at rt_scene_create */
     uint64_t shadow_rays;         /* shadow rays among `rays` */
     uint64_t primary_rays;        /* primary rays among `rays` */
+    uint64_t cache_queries;       /* radiance-cache queries (rt_render_pt_cached) */
 } rt_stats;
 
 int rt_scene_stats(const rt_scene *scene, rt_stats *out);

@@ -4,7 +4,11 @@
 // (src/renderer.cpp:186-229), DepthBufferRenderer::Render (:138-174),
 // AlbedoRenderer::Render (:391-427) and IntersectTestRenderer::Render (:20-54).
 // Each wave owns an 8x8 pixel tile at a time (coherent primary rays share BVH
-// nodes); tiles come from the XCD-banded queue in dev_trace.h.
+// nodes).  Tiles are cut into 8 contiguous bands, band b drained first by the
+// workgroups on XCD b (round-robin dispatch puts workgroup g on XCD g % 8)
+// through one counter per band, then helped by the others: one atomic per
+// tile and no static dealing, which measured faster than BandSched for these
+// 32k coherent tiles per 1080p frame.
 #include "dev_trace.h"
 #include "kernels.h"
 
@@ -24,8 +28,8 @@ __global__ __launch_bounds__(256) void aov_kernel(DevScene sc, rt_camera cam, Ao
     const float divW = 1.0f / float(cam.width);
     const float divH = 1.0f / float(cam.height);
     const float div_fmn = 1.0f / (cam.far_clip - cam.near_clip);
-    const uint32_t xcd = blockIdx.x & 7u;
     uint32_t n_nodes = 0, n_tris = 0;
+    const uint32_t xcd = blockIdx.x & 7u;
     for (uint32_t k = 0; k < 8; ++k) {
         const uint32_t band = (xcd + k) & 7u;
         const uint32_t b0 = band_begin(ntiles, band), b1 = band_begin(ntiles, band + 1);
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(256) void aov_kernel(DevScene sc, rt_camera cam, Ao
     }
 }
 
-// BVH::Intersect over an explicit ray batch; waves fetch 64 rays at a time.
+// BVH::Intersect over an explicit ray batch; a wave's work item is 64 rays.
 __global__ __launch_bounds__(256) void ray_kernel(DevScene sc, const float *src, const float *dir,
                                                   const uint32_t *exclude, float4 *out, uint32_t n,
                                                   uint32_t *fetch, uint32_t *ovf_ref, float *ovf_t,
@@ -93,12 +97,11 @@ __global__ __launch_bounds__(256) void ray_kernel(DevScene sc, const float *src,
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
     Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
     uint32_t nn = 0, nt = 0;
+    BandSched bs = band_sched((n + 63u) / 64u, fetch, sc.share_rays);
     while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
-        const uint32_t i = base + lane;
+        const uint32_t chunk = band_next(bs);
+        if (chunk == kNone) break;
+        const uint32_t i = chunk * 64u + lane;
         if (i >= n) continue;
         const V3 o = {src[3 * (size_t)i], src[3 * (size_t)i + 1], src[3 * (size_t)i + 2]};
         const V3 d = {dir[3 * (size_t)i], dir[3 * (size_t)i + 1], dir[3 * (size_t)i + 2]};

@@ -279,6 +279,7 @@ int rt_scene_create(const rt_scene_desc *desc, rt_scene **out) {
             ds.sky[k] = desc->sky_emission[k];
             ds.ground[k] = desc->ground_reflection[k];
         }
+        ds.share_rays = ds.share_prim = kSchedAuto;
         s->num_tris = n;
         s->bvh_nodes = (uint32_t)bvh.nodes.size();
         s->bvh_depth = bvh.depth;
@@ -376,6 +377,7 @@ int rt_render_aov(rt_scene *s, const rt_camera *cam, uint32_t passes, const rt_a
         st.samples = np;
         st.shadow_rays = 0;
         st.primary_rays = np;
+        st.cache_queries = 0;
         if (count) {
             unsigned long long c[2];
             RT_HIP(hipMemcpy(c, s->d_stat, sizeof(c), hipMemcpyDeviceToHost));
@@ -441,6 +443,7 @@ int rt_intersect(rt_scene *s, uint32_t n, const float *src, const float *dir, co
         s->stats.samples = 0;
         s->stats.shadow_rays = 0;
         s->stats.primary_rays = 0;
+        s->stats.cache_queries = 0;
         return (int)RT_OK;
     });
 }

@@ -114,6 +114,10 @@ struct DevScene {
     uint32_t num_emit;
     uint32_t num_tris;
     float sky[3], ground[3];
+    // launch policy: statically dealt share (eighths, or kSchedAuto) of each
+    // band's chunks for explicit ray batches and path-tracer primary tiles
+    // (see BandSched; the trace queues' share is in PtDev)
+    uint32_t share_rays, share_prim;
 };
 
 struct Hit {
@@ -278,17 +282,86 @@ __device__ __forceinline__ V3 shading_normal(const DevScene &sc, uint32_t id, fl
     return {g.x, g.y, g.z};
 }
 
-// XCD-aware work distribution: the tile range is cut into 8 contiguous bands,
-// band k first served to the workgroups that land on XCD k (round-robin
-// dispatch puts workgroup b on XCD b % 8); a drained band is then helped by
-// the others.  One atomic per wave per tile.
-struct TileQueue {
-    uint32_t *counters;   // 8 counters, each on its own 128-byte line (stride 32 words)
-    uint32_t ntiles;
-};
-
 __device__ __forceinline__ uint32_t band_begin(uint32_t ntiles, uint32_t b) {
     return (uint32_t)(((uint64_t)ntiles * b) / 8u);
+}
+
+// XCD-aware work distribution of a persistent launch over n chunks (a chunk is
+// one wave's work item: 64 rays or an 8x8 tile).  The chunk range is cut into
+// 8 contiguous bands; band b belongs to the workgroups that land on XCD b
+// (round-robin dispatch puts workgroup g on XCD g % 8), so neighbouring chunks
+// share one L2.  Within a band a leading part is dealt statically -- wave i of
+// the band takes chunks i, i + W_b, ... (no atomics) -- and the rest is drained
+// through the band's counter, one atomic per chunk.  A wave whose band is
+// drained helps the other bands, reading a counter before it spends an atomic
+// on it.  Same-address atomics serialise at the memory side, so small launches
+// go all-static (kSchedAuto) and large ones trade atomics for load balance.
+struct BandSched {
+    uint32_t *ctr;         // 8 counters, stride 32 words (own 128-byte lines), zero at launch
+    uint32_t n;            // chunks
+    uint32_t band;         // this wave's XCD band
+    uint32_t wb;           // this wave's index within its band
+    uint32_t k, kb;        // static chunks taken, bands tried
+    uint32_t share;        // statically dealt share of a band, in eighths (0: all dynamic)
+};
+
+__device__ __forceinline__ uint32_t band_waves(uint32_t b) {
+    return ((gridDim.x + 7u - b) >> 3) * (blockDim.x >> 6);
+}
+
+// share: eighths of a band dealt statically, or kSchedAuto = all of it when
+// the band holds at most kSchedAutoRounds chunks per wave, else none.
+constexpr uint32_t kSchedAuto = 0xFFu;
+constexpr uint32_t kSchedAutoRounds = 4u;
+
+// Statically dealt chunks of a band of nb chunks served by Wb waves.
+__device__ __forceinline__ uint32_t band_static(uint32_t nb, uint32_t Wb, uint32_t share) {
+    if (Wb == 0u || share == 0u) return 0u;
+    if (share == kSchedAuto) return nb <= kSchedAutoRounds * Wb ? nb : 0u;
+    if (nb <= Wb) return nb;
+    const uint32_t h = (uint32_t)(((uint64_t)nb * (share > 8u ? 8u : share)) >> 3);
+    return h > Wb ? h : Wb;
+}
+
+__device__ __forceinline__ BandSched band_sched(uint32_t n, uint32_t *ctr, uint32_t share) {
+    BandSched s;
+    s.share = share;
+    s.ctr = ctr;
+    s.n = n;
+    s.band = blockIdx.x & 7u;
+    s.wb = (blockIdx.x >> 3) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    s.k = 0;
+    s.kb = 0;
+    return s;
+}
+
+// Next chunk of this wave (wave-uniform), or kNone when all bands are drained.
+__device__ __forceinline__ uint32_t band_next(BandSched &s) {
+    const uint32_t lane = threadIdx.x & 63u;
+    {
+        const uint32_t lo = band_begin(s.n, s.band), nb = band_begin(s.n, s.band + 1u) - lo;
+        const uint32_t Wb = band_waves(s.band);
+        const uint32_t c = s.wb + s.k * Wb;
+        if (c < band_static(nb, Wb, s.share)) {
+            ++s.k;
+            return lo + c;
+        }
+    }
+    while (s.kb < 8u) {
+        const uint32_t b = (s.band + s.kb) & 7u;
+        const uint32_t lo = band_begin(s.n, b), nb = band_begin(s.n, b + 1u) - lo;
+        const uint32_t st = band_static(nb, band_waves(b), s.share);
+        const uint32_t dn = nb - st;
+        uint32_t t = dn;
+        // own band: straight to the atomic; others: only if not seen drained
+        if (lane == 0u && dn != 0u &&
+            (s.kb == 0u || __hip_atomic_load(s.ctr + 32u * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < dn))
+            t = atomicAdd(s.ctr + 32u * b, 1u);
+        t = __shfl(t, 0);
+        if (t < dn) return lo + st + t;
+        ++s.kb;
+    }
+    return 0xFFFFFFFFu;
 }
 
 }  // namespace rtamd

@@ -531,6 +531,22 @@ __global__ void nrc_zero_kernel(float *p, size_t n) {
 
 // ---------------------------------------------------------------- training paths
 
+// Wave-aggregated append of n items per lane (all 64 lanes active): one atomic
+// per wave; returns this lane's first index.
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t *counter, uint32_t n) {
+    const int lane = (int)(threadIdx.x & 63u);
+    uint32_t x = n;   // inclusive prefix sum over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    const uint32_t total = __shfl(x, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total != 0u) base = atomicAdd(counter, total);
+    base = __shfl(base, 63);
+    return base + x - n;
+}
+
 // Plain paths (the renderer's estimator) from random pixels; each vertex k >=
 // first_vertex becomes a sample: its features and the radiance the renderer
 // adds after vertex k's emission term, dl_k + w_k (e_{k+1} + R_{k+1}).
@@ -543,105 +559,108 @@ __global__ __launch_bounds__(256) void nrc_gen_kernel(DevScene sc, NrcGen gp, fl
     Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
     uint32_t nn = 0, ntt = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t pth = (uint32_t)gtid; pth < gp.npaths; pth += stride) {
-        const uint4 r0 = philox(make_uint4(pth, gp.frame, 0xFFFFFFF0u, 0u), gp.key0, gp.key1);
-        const uint32_t pixel = pick(r0.x, gp.npix);
-        const uint32_t sample = r0.y;   // a sample stream disjoint from rendering in practice
-        V3 o, d;
-        camera_ray(gp.cam, gp.divW, gp.divH, pixel % gp.cam.width, pixel / gp.cam.width, o, d);
+    // whole waves iterate together: the sample append is wave-aggregated
+    for (uint32_t pbase = blockIdx.x * blockDim.x; pbase < gp.npaths; pbase += stride) {
+        const uint32_t pth = pbase + tid;
         float feat[kNrcMaxV][13];
-        V3 ev[kNrcMaxV + 1], dlv[kNrcMaxV], wv[kNrcMaxV];
+        float tgt[kNrcMaxV][3];
         int nv = 0;
-        V3 absn = v3(0.0f);
-        float last_pdf = 0.0f;
-        uint32_t excl = kNone;
-        for (uint32_t dep = 0; dep < gp.max_depth; ++dep) {
-            Hit h;
-            trace<false, false>(sc, o, d, excl, kNone, 0.0f, stk, h, nn, ntt);
-            if (h.id == kNone) {
-                ev[dep] = (-d.z < 0.0f) ? ld3(sc.sky) : ld3(sc.ground);
-                break;
-            }
-            const uint4 ra = philox(make_uint4(pixel, 0x80000000u | (sample & 0x7FFFFFFFu), dep, 0u), gp.key0,
-                                    gp.key1);
-            const Mat m = load_mat(sc.mats[sc.info[h.id] & 0x7FFFFFFFu]);
-            const V3 pos = add(o, muls(d, h.t));
-            const V3 N = shading_normal(sc, h.id, h.u, h.v);
-            V3 ff = N;
-            float eta;
-            if (dot(d, ff) > 0.0f) {
-                ff = neg(ff);
-                eta = m.ior;
-                absn = v3(0.0f);
-            } else {
-                eta = 1.0f / m.ior;
-            }
-            const float pdf_e = emission_pdf(pos, o, neg(d), N, sc.geo[h.id].w);
-            ev[dep] = dep == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
-            const float len = length(sub(pos, o));
-            const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
-            float *f = feat[dep];
-            f[0] = pos.x; f[1] = pos.y; f[2] = pos.z;
-            f[3] = -d.x; f[4] = -d.y; f[5] = -d.z;
-            f[6] = ff.x; f[7] = ff.y; f[8] = ff.z;
-            f[9] = m.roughness;
-            f[10] = m.base.x; f[11] = m.base.y; f[12] = m.base.z;
-            V3 dl = v3(0.0f);
-            if (sc.num_emit > 0) {
-                const EmitterRec &E = sc.emit[pick(ra.x, sc.num_emit)];
-                const float sqr1 = sqrtf(u01(ra.y));
-                const float r2 = u01(ra.z);
-                const V3 epos = add(add(muls(ld3(E.e1), 1.0f - sqr1), muls(ld3(E.e2), (1.0f - r2) * sqr1)), ld3(E.v0));
-                const V3 dir_to = normalized(sub(epos, pos));
-                float te = 0.0f, ue = 0.0f, ve = 0.0f, tmax = kFloatInfinity;
-                if (E.tri != h.id && mt_intersect(pos, dir_to, ld3(E.v0), ld3(E.e1), ld3(E.e2), te, ue, ve) &&
-                    te < kFloatInfinity)
-                    tmax = te;
-                else
-                    ue = ve = 0.0f;
-                const bool esmooth = (sc.info[E.tri] & 0x80000000u) != 0u;
-                const V3 en = esmooth ? shading_normal(sc, E.tri, ue, ve) : ld3(E.normal);
-                const float lightpdf = emission_pdf(epos, pos, neg(dir_to), en, E.area);
-                float bsdfpdf;
-                const V3 fb = BsdfEval(m, eta, neg(d), ff, dir_to, bsdfpdf);
-                if (bsdfpdf > 0.0f) {
-                    Hit sh;
-                    const bool occ = trace<true, false>(sc, pos, dir_to, h.id, E.tri, tmax, stk, sh, nn, ntt);
-                    if (!occ) {
-                        const V3 em = ld3(sc.mats[sc.info[E.tri] & 0x7FFFFFFFu].emission);
-                        dl = mulv(divs(muls(mulv(muls(fb, PowerHeuristic(lightpdf, bsdfpdf)), em), (float)sc.num_emit),
-                                       lightpdf), br);
+        if (pth < gp.npaths) {
+            const uint4 r0 = philox(make_uint4(pth, gp.frame, 0xFFFFFFF0u, 0u), gp.key0, gp.key1);
+            const uint32_t pixel = pick(r0.x, gp.npix);
+            const uint32_t sample = r0.y;   // a sample stream disjoint from rendering in practice
+            V3 o, d;
+            camera_ray(gp.cam, gp.divW, gp.divH, pixel % gp.cam.width, pixel / gp.cam.width, o, d);
+            V3 ev[kNrcMaxV + 1], dlv[kNrcMaxV], wv[kNrcMaxV];
+            V3 absn = v3(0.0f);
+            float last_pdf = 0.0f;
+            uint32_t excl = kNone;
+            for (uint32_t dep = 0; dep < gp.max_depth; ++dep) {
+                Hit h;
+                trace<false, false>(sc, o, d, excl, kNone, 0.0f, stk, h, nn, ntt);
+                if (h.id == kNone) {
+                    ev[dep] = (-d.z < 0.0f) ? ld3(sc.sky) : ld3(sc.ground);
+                    break;
+                }
+                const uint4 ra = philox(make_uint4(pixel, 0x80000000u | (sample & 0x7FFFFFFFu), dep, 0u), gp.key0,
+                                        gp.key1);
+                const Mat m = load_mat(sc.mats[sc.info[h.id] & 0x7FFFFFFFu]);
+                const V3 pos = add(o, muls(d, h.t));
+                const V3 N = shading_normal(sc, h.id, h.u, h.v);
+                V3 ff = N;
+                float eta;
+                if (dot(d, ff) > 0.0f) {
+                    ff = neg(ff);
+                    eta = m.ior;
+                    absn = v3(0.0f);
+                } else {
+                    eta = 1.0f / m.ior;
+                }
+                const float pdf_e = emission_pdf(pos, o, neg(d), N, sc.geo[h.id].w);
+                ev[dep] = dep == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
+                const float len = length(sub(pos, o));
+                const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
+                float *f = feat[dep];
+                f[0] = pos.x; f[1] = pos.y; f[2] = pos.z;
+                f[3] = -d.x; f[4] = -d.y; f[5] = -d.z;
+                f[6] = ff.x; f[7] = ff.y; f[8] = ff.z;
+                f[9] = m.roughness;
+                f[10] = m.base.x; f[11] = m.base.y; f[12] = m.base.z;
+                V3 dl = v3(0.0f);
+                if (sc.num_emit > 0) {
+                    const EmitterRec &E = sc.emit[pick(ra.x, sc.num_emit)];
+                    const float sqr1 = sqrtf(u01(ra.y));
+                    const float r2 = u01(ra.z);
+                    const V3 epos = add(add(muls(ld3(E.e1), 1.0f - sqr1), muls(ld3(E.e2), (1.0f - r2) * sqr1)), ld3(E.v0));
+                    const V3 dir_to = normalized(sub(epos, pos));
+                    float te = 0.0f, ue = 0.0f, ve = 0.0f, tmax = kFloatInfinity;
+                    if (E.tri != h.id && mt_intersect(pos, dir_to, ld3(E.v0), ld3(E.e1), ld3(E.e2), te, ue, ve) &&
+                        te < kFloatInfinity)
+                        tmax = te;
+                    else
+                        ue = ve = 0.0f;
+                    const bool esmooth = (sc.info[E.tri] & 0x80000000u) != 0u;
+                    const V3 en = esmooth ? shading_normal(sc, E.tri, ue, ve) : ld3(E.normal);
+                    const float lightpdf = emission_pdf(epos, pos, neg(dir_to), en, E.area);
+                    float bsdfpdf;
+                    const V3 fb = BsdfEval(m, eta, neg(d), ff, dir_to, bsdfpdf);
+                    if (bsdfpdf > 0.0f) {
+                        Hit sh;
+                        const bool occ = trace<true, false>(sc, pos, dir_to, h.id, E.tri, tmax, stk, sh, nn, ntt);
+                        if (!occ) {
+                            const V3 em = ld3(sc.mats[sc.info[E.tri] & 0x7FFFFFFFu].emission);
+                            dl = mulv(divs(muls(mulv(muls(fb, PowerHeuristic(lightpdf, bsdfpdf)), em), (float)sc.num_emit),
+                                           lightpdf), br);
+                        }
                     }
                 }
+                dlv[dep] = dl;
+                nv = dep + 1;
+                ev[dep + 1] = v3(0.0f);
+                wv[dep] = v3(0.0f);
+                if (dep + 1 >= gp.max_depth) break;
+                const uint4 rb = philox(make_uint4(pixel, 0x80000000u | (sample & 0x7FFFFFFFu), dep, 1u), gp.key0, gp.key1);
+                V3 nd, color;
+                float bpdf;
+                if (!BsdfSample(m, eta, ff, neg(d), u01(ra.w), u01(rb.x), nd, color, bpdf)) break;
+                absn = divs(neg(V3{logf(m.extinction.x), logf(m.extinction.y), logf(m.extinction.z)}), m.at_distance);
+                if (!(bpdf > 0.0f)) break;
+                wv[dep] = divs(mulv(br, color), bpdf);
+                o = pos;
+                d = nd;
+                excl = h.id;
+                last_pdf = bpdf;
             }
-            dlv[dep] = dl;
-            nv = dep + 1;
-            ev[dep + 1] = v3(0.0f);
-            wv[dep] = v3(0.0f);
-            if (dep + 1 >= gp.max_depth) break;
-            const uint4 rb = philox(make_uint4(pixel, 0x80000000u | (sample & 0x7FFFFFFFu), dep, 1u), gp.key0, gp.key1);
-            V3 nd, color;
-            float bpdf;
-            if (!BsdfSample(m, eta, ff, neg(d), u01(ra.w), u01(rb.x), nd, color, bpdf)) break;
-            absn = divs(neg(V3{logf(m.extinction.x), logf(m.extinction.y), logf(m.extinction.z)}), m.at_distance);
-            if (!(bpdf > 0.0f)) break;
-            wv[dep] = divs(mulv(br, color), bpdf);
-            o = pos;
-            d = nd;
-            excl = h.id;
-            last_pdf = bpdf;
-        }
-        // targets, last vertex first
-        V3 R = v3(0.0f);
-        float tgt[kNrcMaxV][3];
-        for (int k = nv - 1; k >= 0; --k) {
-            R = add(dlv[k], mulv(wv[k], add(ev[k + 1], R)));
-            tgt[k][0] = R.x; tgt[k][1] = R.y; tgt[k][2] = R.z;
+            // targets, last vertex first
+            V3 R = v3(0.0f);
+            for (int k = nv - 1; k >= 0; --k) {
+                R = add(dlv[k], mulv(wv[k], add(ev[k + 1], R)));
+                tgt[k][0] = R.x; tgt[k][1] = R.y; tgt[k][2] = R.z;
+            }
         }
         const int first = (int)gp.first_vertex;
-        const int ns = nv > first ? nv - first : 0;
-        if (ns == 0) continue;
-        const uint32_t base = atomicAdd(count, (uint32_t)ns);
+        const uint32_t ns = nv > first ? (uint32_t)(nv - first) : 0u;
+        const uint32_t base = wave_append_n(count, ns);
         for (int k = first; k < nv; ++k) {
             const uint32_t slot = base + (uint32_t)(k - first);
             if (slot >= gp.capacity) break;

@@ -31,17 +31,36 @@ static uint32_t *ctr_at_host(const PtDev &p, uint32_t batch, uint32_t d, uint32_
     return p.ctr + (((size_t)batch * kMaxDepth + d) * 4u + k) * 32u;
 }
 
-// Wave-aggregated append: one atomic per wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
-    const uint64_t mask = __ballot(pred);
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t base = 0;
-    const uint32_t total = (uint32_t)__popcll(mask);
-    const uint32_t leader = mask ? (uint32_t)(__ffsll((unsigned long long)mask) - 1) : 0u;
-    if (lane == leader && total) base = atomicAdd(counter, total);
-    base = __shfl(base, (int)leader);
-    const uint64_t below = mask & ((1ull << lane) - 1ull);
-    return base + (uint32_t)__popcll(below);
+// Block-aggregated appends to three queues: one atomic per queue per block and
+// call instead of one per wave (same-address atomics serialise).  Every thread
+// of the block makes the call (block-uniform control flow); consecutive calls
+// alternate between two LDS buffers, so one barrier pair per call suffices.
+struct AppendLds {
+    uint32_t cnt[2][3][8];
+    uint32_t base[2][3];
+};
+
+__device__ __forceinline__ void block_append3(AppendLds &L, uint32_t it, uint32_t *c0, bool p0, uint32_t *c1,
+                                              bool p1, uint32_t *c2, bool p2, uint32_t idx[3]) {
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6, b = it & 1u;
+    const uint64_t m[3] = {__ballot(p0), __ballot(p1), __ballot(p2)};
+    if (lane == 0u)
+        for (int q = 0; q < 3; ++q) L.cnt[b][q][w] = (uint32_t)__popcll(m[q]);
+    __syncthreads();
+    if (threadIdx.x < 3u) {
+        const uint32_t q = threadIdx.x;
+        uint32_t sum = 0;
+        for (uint32_t i = 0; i < nw; ++i) sum += L.cnt[b][q][i];
+        uint32_t *c = q == 0u ? c0 : (q == 1u ? c1 : c2);
+        L.base[b][q] = (sum != 0u && c != nullptr) ? atomicAdd(c, sum) : 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int q = 0; q < 3; ++q) {
+        uint32_t off = L.base[b][q];
+        for (uint32_t i = 0; i < w; ++i) off += L.cnt[b][q][i];
+        idx[q] = off + (uint32_t)__popcll(m[q] & below);
+    }
 }
 
 __global__ __launch_bounds__(256) void pt_primary_kernel(DevScene sc, PtDev p, uint32_t tiles_x, uint32_t ntiles,
@@ -51,17 +70,12 @@ __global__ __launch_bounds__(256) void pt_primary_kernel(DevScene sc, PtDev p, u
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
     Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
-    const uint32_t xcd = blockIdx.x & 7u;
     uint32_t nn = 0, nt = 0;
-    for (uint32_t k = 0; k < 8; ++k) {
-        const uint32_t band = (xcd + k) & 7u;
-        const uint32_t b0 = band_begin(ntiles, band), b1 = band_begin(ntiles, band + 1);
+    BandSched bs = band_sched(ntiles, p.prim_ctr, sc.share_prim);
+    {
         while (true) {
-            uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(p.prim_ctr + band * 32u, 1u);
-            t = __shfl(t, 0);
-            if (t >= b1 - b0) break;
-            const uint32_t tile = b0 + t;
+            const uint32_t tile = band_next(bs);
+            if (tile == kNone) break;
             const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u);
             const uint32_t y = (tile / tiles_x) * 8u + (lane >> 3);
             if (x >= p.cam.width || y >= p.cam.height) continue;
@@ -85,8 +99,11 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
     const uint32_t *q_in = (depth & 1u) ? p.q1 : p.q0;
     uint32_t *next_ctr = last ? nullptr : ctr_at(p, batch, depth + 1, 0);
     uint32_t *sh_ctr = ctr_at(p, batch, depth, 2);
-    // Iterate whole waves so the wave-level appends see every lane.
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    uint32_t *q_ctr = p.cache_depth ? ctr_at(p, batch, depth, 3) : nullptr;
+    __shared__ AppendLds s_app;
+    uint32_t it = 0;
+    // Iterate whole blocks so the block-level appends see every lane.
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride, ++it) {
         const uint32_t j = base + threadIdx.x;
         const bool active = j < count;
         bool push_next = false, push_shadow = false, push_query = false;
@@ -213,14 +230,13 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
             }
             p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
         }
-        const uint32_t qi = wave_append(next_ctr ? next_ctr : sh_ctr, push_next && next_ctr);
-        if (push_next && next_ctr) q_out[qi] = slot;
-        const uint32_t si = wave_append(sh_ctr, push_shadow);
-        if (push_shadow) p.sq[si] = slot;
-        if (p.cache_depth) {
-            const uint32_t ci = wave_append(ctr_at(p, batch, depth, 3), push_query);
+        uint32_t idx[3];
+        block_append3(s_app, it, next_ctr, push_next && next_ctr, sh_ctr, push_shadow, q_ctr, push_query, idx);
+        if (push_next && next_ctr) q_out[idx[0]] = slot;
+        if (push_shadow) p.sq[idx[1]] = slot;
+        {
             if (push_query) {
-                float *dst = p.cache_q + kNrcFloats * (size_t)ci;
+                float *dst = p.cache_q + kNrcFloats * (size_t)idx[2];
                 for (int c = 0; c < 13; ++c) dst[c] = qf[c];
                 dst[kRecBounces] = (float)(p.max_depth - 1u - depth);
                 dst[kRecBounces + 1] = 0.0f;
@@ -239,15 +255,14 @@ __global__ __launch_bounds__(256) void pt_trace_kernel(DevScene sc, PtDev p, uin
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
     Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
     const uint32_t count = *ctr_at(p, batch, depth, SHADOW ? 2 : 0);
-    uint32_t *fetch = ctr_at(p, batch, depth, SHADOW ? 3 : 1);
     const uint32_t *q = SHADOW ? p.sq : ((depth & 1u) ? p.q1 : p.q0);
     uint32_t nn = 0, nt = 0;
+    BandSched bs = band_sched((count + 63u) / 64u, p.fctr + (depth * 2u + (SHADOW ? 1u : 0u)) * 8u * 32u,
+                              p.share_trace);
     while (true) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        const uint32_t j = base + lane;
+        const uint32_t chunk = band_next(bs);
+        if (chunk == kNone) break;
+        const uint32_t j = chunk * 64u + lane;
         if (j >= count) continue;
         const uint32_t slot = q[j];
         const float4 ro = p.ray_o[slot];
@@ -328,6 +343,8 @@ hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, 
                             int64_t ovf_stride, const PtCache &cache, hipStream_t s) {
     const uint32_t npaths = K * p.npix;
     const uint32_t depth_end = p.cache_depth ? p.cache_depth + 1 : p.max_depth;
+    hipLaunchKernelGGL(pt_zero_u32_kernel, dim3((unsigned)((kFetchWords + 255) / 256)), dim3(256), 0, s, p.fctr,
+                       kFetchWords);
     for (uint32_t d = 0; d < depth_end; ++d) {
         if (d == 0) {
             hipLaunchKernelGGL(pt_shade_kernel<true>, dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0, npaths,

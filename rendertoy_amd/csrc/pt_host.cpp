@@ -31,9 +31,19 @@ struct PtState {
     hipGraph_t graph = nullptr;
     std::vector<uint8_t> key;
     uint32_t batches = 0, K = 0;
-    float *cache_q = nullptr;       // radiance-cache queries (cap x 16 floats)
+    float *cache_q = nullptr;       // radiance-cache queries (cap x record floats)
     size_t cache_q_cap = 0;
+    // launch-policy tuner: the first renders of a workload shape try each
+    // trace-queue policy once and keep the fastest (the image does not depend on it)
+    uint64_t tune_key[4] = {0, 0, 0, 0};
+    uint32_t tune_trial = 0, tune_best = 0;
+    float tune_ms[4] = {0, 0, 0, 0};
 };
+
+// Trace-queue policies tried by the tuner: all-static for small queues (no
+// atomics; best when ray costs are uniform), and all-dynamic (best when they vary).
+static const uint32_t kTraceShares[] = {kSchedAuto, 0u};
+constexpr uint32_t kNumTraceShares = sizeof(kTraceShares) / sizeof(kTraceShares[0]);
 
 namespace {
 
@@ -137,6 +147,7 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
             p.fb = (float *)alloc(st, (size_t)npix * 12);
             p.ctr = (uint32_t *)alloc(st, ctr_words * 4);
             p.prim_ctr = (uint32_t *)alloc(st, 256 * 4);
+            p.fctr = (uint32_t *)alloc(st, kFetchWords * 4);
             st->cap = cap;
             st->npix = npix;
             st->ctr_words = ctr_words;
@@ -146,6 +157,20 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
             st->cache_q_cap = cap;
         }
         PtDev &p = st->dev;
+        // trace-queue policy: fixed, or the tuner's trial / choice for this workload shape
+        uint32_t trial = kNumTraceShares;
+        if (prm->flags & RT_PT_NO_TUNE) {
+            p.share_trace = kSchedAuto;
+        } else {
+            const uint64_t tk[4] = {npix, ((uint64_t)prm->spp << 32) | prm->max_depth, (uint64_t)(uintptr_t)cache,
+                                    ((uint64_t)K << 32) | (prm->flags & ~(uint32_t)RT_PT_NO_GRAPH)};
+            if (std::memcmp(tk, st->tune_key, sizeof(tk)) != 0) {
+                std::memcpy(st->tune_key, tk, sizeof(tk));
+                st->tune_trial = 0;
+            }
+            if (st->tune_trial < kNumTraceShares) trial = st->tune_trial;
+            p.share_trace = kTraceShares[trial < kNumTraceShares ? trial : st->tune_best];
+        }
         p.cam = *cam;
         p.cache_depth = cache ? cache->prm.first_vertex : 0;
         p.cache_q = cache ? st->cache_q : nullptr;
@@ -218,16 +243,27 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
         hipchk(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]), "hipEventElapsedTime");
         st->batches = nb;
         st->K = K;
+        if (trial < kNumTraceShares) {
+            st->tune_ms[trial] = ms;
+            st->tune_trial = trial + 1;
+            if (st->tune_trial == kNumTraceShares) {
+                uint32_t best = 0;
+                for (uint32_t i = 1; i < kNumTraceShares; ++i)
+                    if (st->tune_ms[i] < st->tune_ms[best]) best = i;
+                st->tune_best = best;
+            }
+        }
 
         // ray counts from the per-batch counters
         std::vector<uint32_t> ctr(ctr_words);
         hipchk(hipMemcpy(ctr.data(), p.ctr, ctr_words * 4, hipMemcpyDeviceToHost), "hipMemcpy");
-        uint64_t rays = npix, shadow = 0;
+        uint64_t rays = npix, shadow = 0, queries = 0;
         for (uint32_t b = 0; b < nb; ++b)
             for (uint32_t d = 0; d < prm->max_depth; ++d) {
                 const uint32_t *c = ctr.data() + (((size_t)b * kMaxDepth + d) * 4) * 32;
                 if (d > 0) rays += c[0];
                 shadow += c[2 * 32];
+                if (cache && d == p.cache_depth) queries += c[3 * 32];
             }
         rays += shadow;
         rt_stats &sts = s->stats;
@@ -237,6 +273,7 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
         sts.shadow_rays = shadow;
         sts.primary_rays = npix;
         sts.samples = (uint64_t)npix * prm->spp;
+        sts.cache_queries = queries;
         sts.node_visits = sts.tri_tests = 0;
         if (rgb_host) hipchk(hipMemcpy(rgb_host, p.fb, (size_t)npix * 12, hipMemcpyDeviceToHost), "hipMemcpy");
         return RT_OK;

@@ -18,6 +18,7 @@ struct DevScene;
 
 constexpr uint32_t kMaxDepth = 64;      // path vertices
 constexpr uint32_t kMaxBatches = 4096;
+constexpr size_t kFetchWords = (size_t)kMaxDepth * 2 * 8 * 32;
 
 struct PtDev {
     rt_camera cam;
@@ -28,6 +29,9 @@ struct PtDev {
     uint32_t *q0, *q1, *sq;
     uint32_t *ctr;        // kMaxDepth * 4 counters per batch, 32-word stride
     uint32_t *prim_ctr;   // 8 band counters, 32-word stride
+    uint32_t *fctr;       // trace-kernel band counters: [kMaxDepth][extend, shadow][8], 32-word stride,
+                          // cleared at the start of each batch
+    uint32_t share_trace; // trace-queue scheduling policy (BandSched share)
     float *fb;            // npix * 3
     // radiance cache: paths reaching vertex cache_depth (>= 1; 0 = off) add their
     // emission term there and leave a cache query (16 floats, slot in [13]).

@@ -46,7 +46,8 @@ def test_pt_matches_oracle(orc, name, spp, depth):
 
 def test_pt_deterministic_across_launch_shapes():
     """Same seed: bit-identical frames for different batch sizes, with and
-    without the hipGraph, and on a second replay of the graph."""
+    without the hipGraph, on a second replay of the graph, and under every
+    trace-queue policy (the first two renders are the tuner's trials)."""
     from rendertoy_amd import capi
     sc = scene("cornell")
     s = capi.Scene(sc)
@@ -54,7 +55,9 @@ def test_pt_deterministic_across_launch_shapes():
     b = s.render_pt(spp=8, max_depth=5, seed=7)                       # graph replay
     c = s.render_pt(spp=8, max_depth=5, seed=7, flags=capi.PT_NO_GRAPH)
     d = s.render_pt(spp=8, max_depth=5, seed=7, batch_paths=64 * 64 * 3)   # batches of 3 samples
-    for x in (b, c, d):
+    f = s.render_pt(spp=8, max_depth=5, seed=7, flags=capi.PT_NO_TUNE)
+    g = s.render_pt(spp=8, max_depth=5, seed=7)                       # the tuner's choice
+    for x in (b, c, d, f, g):
         assert np.array_equal(a.view(np.uint32), x.view(np.uint32))
     e = s.render_pt(spp=8, max_depth=5, seed=8)
     assert not np.array_equal(a, e)

@@ -100,6 +100,38 @@ def cpu_baseline_c0(sc, spp):
                 sc.width, sc.height, spp)}
 
 
+def bench_c3(n_frames=2):
+    """64 terrain scenes (10k-500k triangles, 128^2..1024^2, 32 spp, depth 6)
+    in one batched wavefront, against the same frames rendered one scene at a time."""
+    from rendertoy_amd import capi, scenes
+    scs = scenes.batch_c3()
+    ss = [capi.Scene(sc) for sc in scs]
+    samples = sum(sc.width * sc.height * sc.spp for sc in scs)
+    b = capi.Batch(ss)
+    for _ in range(3):
+        b.render(spp=32, max_depth=6, download=False)
+    t0 = time.perf_counter()
+    rays = 0
+    for i in range(n_frames):
+        b.render(spp=32, max_depth=6, seed=i, download=False)
+        rays += b.stats()["rays"]
+    dt = (time.perf_counter() - t0) / n_frames
+    for s in ss:
+        for _ in range(2):
+            s.render_pt(spp=32, max_depth=6, download=False)
+    t0 = time.perf_counter()
+    for s in ss:
+        s.render_pt(spp=32, max_depth=6, download=False)
+    seq = time.perf_counter() - t0
+    b.close()
+    return {"workload": "64 terrain scenes, %d triangles in all (10k-500k each, log-uniform), square frames of "
+                        "128-1024 px, 32 spp, depth 6, one batched wavefront" % sum(sc.num_triangles for sc in scs),
+            "samples_per_frame": samples, "msamples_per_s": round(samples / dt / 1e6, 2),
+            "mrays_per_s": round(rays / n_frames / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3, 2),
+            "one_scene_at_a_time_ms": round(seq * 1e3, 2),
+            "one_scene_at_a_time_msamples_per_s": round(samples / seq / 1e6, 2)}
+
+
 def _rmse(img, ref, ok):
     d = img.astype(np.float64)[ok] - ref[ok]
     return float(np.sqrt(np.mean(d * d)))
@@ -328,6 +360,7 @@ def main():
                             "msamples_per_s": round(sc2.width * sc2.height * sc2.spp * n2 / dt / 1e6, 2),
                             "mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / n2, 2)}
         del s2
+        pt["c3_batch"] = bench_c3()
         out["path_tracer"] = pt
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_c1(sc, args.cpu_seconds)

@@ -133,7 +133,8 @@ typedef struct rt_pt_params {
     uint32_t spp;
     uint32_t max_depth;
     uint64_t seed;
-    uint32_t batch_paths;   /* paths in flight per wavefront batch; 0 = automatic */
+    uint32_t batch_paths;   /* paths in flight per wavefront batch; 0 = automatic (up to 64M, at
+                               most an eighth of the free device memory) */
     uint32_t flags;         /* RT_PT_* */
 } rt_pt_params;
 
@@ -231,6 +232,28 @@ int rt_scene_stats(const rt_scene *scene, rt_stats *out);
 
 /* Turns per-ray node/triangle counting on or off for later calls (slower). */
 int rt_scene_set_counting(rt_scene *scene, int on);
+
+/* ---------------------------------------------------------------------------
+ * Batched rendering (no counterpart in RenderToy, whose Render() draws one
+ * World): the paths of many scenes go through one wavefront -- the same
+ * bounce-loop launches and one hipGraph -- so small scenes fill the GPU
+ * together.  Scene i's frame is bit-identical to rt_render_pt of that scene
+ * with the same camera and parameters.  The scenes must be on one device and
+ * outlive the batch (destroying one first detaches the batch: every later
+ * call on it but rt_batch_destroy fails with RT_ERR_INVALID).
+ * ------------------------------------------------------------------------- */
+typedef struct rt_batch rt_batch;
+
+int rt_batch_create(rt_scene *const *scenes, uint32_t num_scenes, rt_batch **out);
+int rt_batch_destroy(rt_batch *batch);
+
+/* One frame of every scene: cams[i] is scene i's camera (resolutions may
+ * differ); rgb_host[i] (may be NULL, as may rgb_host itself) receives scene
+ * i's float[width*height][3].  Blocks until done. */
+int rt_batch_render(rt_batch *batch, const rt_camera *cams, const rt_pt_params *params, float *const *rgb_host);
+
+/* Counters and timings of the batch's last frame (all scenes together). */
+int rt_batch_stats(const rt_batch *batch, rt_stats *out);
 
 /* Device pointer of a pass left on the device by the last render, for callers
  * that keep results in HBM (pass = one RT_AOV_* bit, or 0 for the path-traced

@@ -118,6 +118,10 @@ def lib():
     L.rt_cache_set_params.argtypes = [vp, vp, vp]
     L.rt_cache_layout.argtypes = [vp, vp, vp, vp, vp]
     L.rt_render_pt_cached.argtypes = [vp, vp, C.POINTER(rt_camera), C.POINTER(rt_pt_params), vp]
+    L.rt_batch_create.argtypes = [C.POINTER(vp), C.c_uint32, C.POINTER(vp)]
+    L.rt_batch_destroy.argtypes = [vp]
+    L.rt_batch_render.argtypes = [vp, C.POINTER(rt_camera), C.POINTER(rt_pt_params), C.POINTER(vp)]
+    L.rt_batch_stats.argtypes = [vp, C.POINTER(rt_stats)]
     _lib = L
     return L
 
@@ -329,3 +333,41 @@ class Cache:
         check(lib().rt_render_pt_cached(self.scene.h, self.h, C.byref(cam), C.byref(p),
                                         rgb.ctypes.data if download else None))
         return rgb
+
+
+class Batch:
+    """An rt_batch: many Scenes path-traced through one wavefront."""
+
+    def __init__(self, scenes):
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[s.h for s in self.scenes])
+        h = C.c_void_p()
+        check(lib().rt_batch_create(arr, len(self.scenes), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, cams=None, spp=32, max_depth=6, seed=0, batch_paths=0, flags=0, download=True):
+        """Frames of every scene (list of (W*H, 3) arrays, or None without download)."""
+        n = len(self.scenes)
+        cams = cams or [s.camera() for s in self.scenes]
+        carr = (rt_camera * n)(*cams)
+        p = rt_pt_params(spp, max_depth, seed, batch_paths, flags)
+        outs = [np.zeros((c.width * c.height, 3), np.float32) for c in cams] if download else None
+        ptrs = (C.c_void_p * n)(*[o.ctypes.data for o in outs]) if download else None
+        check(lib().rt_batch_render(self.h, carr, C.byref(p), ptrs))
+        return outs
+
+    def stats(self) -> dict:
+        s = rt_stats()
+        check(lib().rt_batch_stats(self.h, C.byref(s)))
+        return s.as_dict()

@@ -133,6 +133,22 @@ int run_guarded_hip(hipError_t e, const char *what) {
     return RT_OK;
 }
 
+rt_scene *make_render_context(int device) {
+    rt_scene *s = new rt_scene();
+    try {
+        RT_HIP(hipSetDevice(device));
+        check_device(s);
+        RT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (auto &e : s->ev) RT_HIP(hipEventCreate(&e));
+        s->ds = DevScene{};
+        s->ds.share_rays = s->ds.share_prim = kSchedAuto;
+    } catch (...) {
+        delete s;
+        throw;
+    }
+    return s;
+}
+
 }  // namespace rtamd
 
 using namespace rtamd;
@@ -306,6 +322,7 @@ int rt_scene_destroy(rt_scene *scene) {
         if (scene) {
             (void)hipStreamSynchronize(scene->stream);
             detach_caches(scene);
+            detach_batches(scene);
             delete scene;
         }
         return (int)RT_OK;

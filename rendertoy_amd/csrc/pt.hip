@@ -15,6 +15,13 @@
 // A path slot i of a batch is pixel i % npix, sample s0 + i / npix.  All
 // randomness is Philox4x32-10 on (pixel, sample, depth, k), so the image does
 // not depend on batch size, queue order or launch shape.
+//
+// Batched mode (BATCH = true) runs the paths of many scenes through the same
+// launches: scene i owns pixels [pix_off_i, pix_off_i + npix_i) of the frame
+// buffers and slots [K pix_off_i, K (pix_off_i + npix_i)) of a batch; a path
+// finds its scene once per batch (binary search over the offsets) and keeps
+// the scene index in sid[slot].  Pixel indices in the random stream are the
+// scene's own, so each scene's frame equals its single-scene render.
 #include "dev_bsdf.h"
 #include "nrc_kernels.h"
 #include "pt_kernels.h"
@@ -63,6 +70,18 @@ __device__ __forceinline__ void block_append3(AppendLds &L, uint32_t it, uint32_
     }
 }
 
+// Largest i < n with off[i] * scale <= x (off[0] = 0, increasing).
+__device__ __forceinline__ uint32_t find_range(const uint32_t *off, uint32_t n, uint32_t scale, uint32_t x) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] * scale <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool BATCH>
 __global__ __launch_bounds__(256) void pt_primary_kernel(DevScene sc, PtDev p, uint32_t tiles_x, uint32_t ntiles,
                                                          uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
     __shared__ uint32_t s_ref[kLdsStack * 256];
@@ -72,10 +91,23 @@ __global__ __launch_bounds__(256) void pt_primary_kernel(DevScene sc, PtDev p, u
     Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
     uint32_t nn = 0, nt = 0;
     BandSched bs = band_sched(ntiles, p.prim_ctr, sc.share_prim);
-    {
-        while (true) {
-            const uint32_t tile = band_next(bs);
-            if (tile == kNone) break;
+    while (true) {
+        const uint32_t tile = band_next(bs);
+        if (tile == kNone) break;
+        if constexpr (BATCH) {
+            // the tile (and so its scene) is wave-uniform
+            const uint32_t si = find_range(p.tile_off, p.nscenes, 1u, tile);
+            const BatchScene &b = p.bscn[si];
+            const uint32_t t = tile - p.tile_off[si];
+            const uint32_t x = (t % b.tiles_x) * 8u + (lane & 7u);
+            const uint32_t y = (t / b.tiles_x) * 8u + (lane >> 3);
+            if (x >= b.cam.width || y >= b.cam.height) continue;
+            V3 o, d;
+            camera_ray(b.cam, b.divW, b.divH, x, y, o, d);
+            Hit h;
+            trace<false, false>(b.sc, o, d, kNone, kNone, 0.0f, stk, h, nn, nt);
+            p.prim[p.pix_off[si] + (size_t)y * b.cam.width + x] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+        } else {
             const uint32_t x = (tile % tiles_x) * 8u + (lane & 7u);
             const uint32_t y = (tile / tiles_x) * 8u + (lane >> 3);
             if (x >= p.cam.width || y >= p.cam.height) continue;
@@ -88,7 +120,133 @@ __global__ __launch_bounds__(256) void pt_primary_kernel(DevScene sc, PtDev p, u
     }
 }
 
+// One path vertex: Radiance (src/renderer.cpp:272-333) up to the next ray.
+// pixel is the scene's own pixel index (camera ray, random stream); gpix its
+// index in the frame buffers.
 template <bool FIRST>
+__device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &cam, float divW, float divH,
+                                           uint32_t gpix, uint32_t pixel, uint32_t sample, const PtDev &p,
+                                           uint32_t slot, uint32_t depth, bool term, bool last, bool &push_next,
+                                           bool &push_shadow, bool &push_query, float qf[13]) {
+    V3 o, d;
+    float last_pdf;
+    uint32_t hid;
+    float ht, hu, hv;
+    V3 T, L, absn;
+    if (FIRST) {
+        camera_ray(cam, divW, divH, pixel % cam.width, pixel / cam.width, o, d);
+        last_pdf = 0.0f;
+        float4 h = p.prim[gpix];
+        ht = h.x; hu = h.y; hv = h.z; hid = __float_as_uint(h.w);
+        T = v3(1.0f);
+        L = v3(0.0f);
+        absn = v3(0.0f);
+    } else {
+        float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
+        float4 tt = p.thr[slot], ll = p.rad[slot], ab = p.absn[slot];
+        o = {ro.x, ro.y, ro.z};
+        last_pdf = ro.w;
+        d = {rd.x, rd.y, rd.z};
+        ht = h.x; hu = h.y; hv = h.z; hid = __float_as_uint(h.w);
+        T = {tt.x, tt.y, tt.z};
+        L = {ll.x, ll.y, ll.z};
+        absn = {ab.x, ab.y, ab.z};
+    }
+    if (hid == kNone) {
+        // World::GetDefaultEmission(-dir) (src/world.cpp:39-42)
+        V3 bg = (-d.z < 0.0f) ? ld3(sc.sky) : ld3(sc.ground);
+        L = add(L, mulv(T, bg));
+    } else {
+        const uint4 ra = philox(make_uint4(pixel, sample, depth, 0u), p.key0, p.key1);
+        const uint32_t minfo = sc.info[hid];
+        const Mat m = load_mat(sc.mats[minfo & 0x7FFFFFFFu]);
+        const V3 pos = add(o, muls(d, ht));
+        const V3 N = shading_normal(sc, hid, hu, hv);
+        V3 ff = N;
+        float eta;
+        if (dot(d, ff) > 0.0f) {
+            ff = neg(ff);
+            eta = m.ior;
+            absn = v3(0.0f);
+        } else {
+            eta = 1.0f / m.ior;
+        }
+        const float area = sc.geo[hid].w;
+        const float pdf_e = emission_pdf(pos, o, neg(d), N, area);
+        V3 e_term = depth == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
+        L = add(L, mulv(T, e_term));
+        if (term) {
+            // the cache stands for everything after this vertex's emission term
+            push_query = true;
+            qf[0] = pos.x; qf[1] = pos.y; qf[2] = pos.z;
+            qf[3] = -d.x; qf[4] = -d.y; qf[5] = -d.z;
+            qf[6] = ff.x; qf[7] = ff.y; qf[8] = ff.z;
+            qf[9] = m.roughness;
+            qf[10] = m.base.x; qf[11] = m.base.y; qf[12] = m.base.z;
+            p.thr[slot] = make_float4(T.x, T.y, T.z, 0.0f);
+        }
+        const float len = length(sub(pos, o));
+        const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
+        // DirectLight (src/renderer.cpp:335-384)
+        if (sc.num_emit > 0 && !term) {
+            const EmitterRec &E = sc.emit[pick(ra.x, sc.num_emit)];
+            const float sqr1 = sqrtf(u01(ra.y));
+            const float r2 = u01(ra.z);
+            const float ea = 1.0f - sqr1, eb = (1.0f - r2) * sqr1;
+            const V3 epos = add(add(muls(ld3(E.e1), ea), muls(ld3(E.e2), eb)), ld3(E.v0));
+            const V3 dir_to = normalized(sub(epos, pos));
+            float te = 0.0f, ue = 0.0f, ve = 0.0f, tmax = kFloatInfinity;
+            if (E.tri != hid && mt_intersect(pos, dir_to, ld3(E.v0), ld3(E.e1), ld3(E.e2), te, ue, ve) &&
+                te < kFloatInfinity) {
+                tmax = te;
+            } else {
+                ue = 0.0f;
+                ve = 0.0f;
+            }
+            const bool esmooth = (sc.info[E.tri] & 0x80000000u) != 0u;
+            const V3 en = esmooth ? shading_normal(sc, E.tri, ue, ve) : ld3(E.normal);
+            const float lightpdf = emission_pdf(epos, pos, neg(dir_to), en, E.area);
+            float bsdfpdf;
+            const V3 f = BsdfEval(m, eta, neg(d), ff, dir_to, bsdfpdf);
+            const float w = PowerHeuristic(lightpdf, bsdfpdf);
+            if (bsdfpdf > 0.0f) {
+                const V3 em = ld3(sc.mats[sc.info[E.tri] & 0x7FFFFFFFu].emission);
+                const V3 c = divs(muls(mulv(muls(f, w), em), (float)sc.num_emit), lightpdf);
+                const V3 contrib = mulv(T, mulv(c, br));
+                if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
+                    push_shadow = true;
+                    p.sh_d[slot] = make_float4(dir_to.x, dir_to.y, dir_to.z, tmax);
+                    p.sh_c[slot] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(E.tri));
+                }
+            }
+        }
+        V3 nd = v3(0.0f);
+        float bpdf = 0.0f;
+        if (!last) {
+            const uint4 rb = philox(make_uint4(pixel, sample, depth, 1u), p.key0, p.key1);
+            V3 color;
+            if (BsdfSample(m, eta, ff, neg(d), u01(ra.w), u01(rb.x), nd, color, bpdf)) {
+                absn = divs(neg(V3{logf(m.extinction.x), logf(m.extinction.y), logf(m.extinction.z)}),
+                            m.at_distance);
+                if (bpdf > 0.0f) {
+                    T = mulv(T, divs(mulv(br, color), bpdf));
+                    push_next = true;
+                }
+            }
+        }
+        if (push_next || push_shadow) {
+            p.ray_o[slot] = make_float4(pos.x, pos.y, pos.z, bpdf);
+            p.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(hid));
+        }
+        if (push_next) {
+            p.thr[slot] = make_float4(T.x, T.y, T.z, 0.0f);
+            p.absn[slot] = make_float4(absn.x, absn.y, absn.z, 0.0f);
+        }
+    }
+    p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+}
+
+template <bool FIRST, bool BATCH>
 __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t s0,
                                                        uint32_t npaths, uint32_t depth) {
     const uint32_t count = FIRST ? npaths : *ctr_at(p, batch, depth, 0);
@@ -111,142 +269,40 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
         uint32_t slot = 0;
         if (active) {
             slot = FIRST ? j : q_in[j];
-            const uint32_t pixel = slot % p.npix;
-            const uint32_t sample = s0 + slot / p.npix;
-            V3 o, d;
-            float last_pdf;
-            uint32_t hid;
-            float ht, hu, hv;
-            V3 T, L, absn;
-            if (FIRST) {
-                camera_ray(p.cam, p.divW, p.divH, pixel % p.cam.width, pixel / p.cam.width, o, d);
-                last_pdf = 0.0f;
-                float4 h = p.prim[pixel];
-                ht = h.x; hu = h.y; hv = h.z; hid = __float_as_uint(h.w);
-                T = v3(1.0f);
-                L = v3(0.0f);
-                absn = v3(0.0f);
-            } else {
-                float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
-                float4 tt = p.thr[slot], ll = p.rad[slot], ab = p.absn[slot];
-                o = {ro.x, ro.y, ro.z};
-                last_pdf = ro.w;
-                d = {rd.x, rd.y, rd.z};
-                ht = h.x; hu = h.y; hv = h.z; hid = __float_as_uint(h.w);
-                T = {tt.x, tt.y, tt.z};
-                L = {ll.x, ll.y, ll.z};
-                absn = {ab.x, ab.y, ab.z};
-            }
-            if (hid == kNone) {
-                // World::GetDefaultEmission(-dir) (src/world.cpp:39-42)
-                V3 bg = (-d.z < 0.0f) ? ld3(sc.sky) : ld3(sc.ground);
-                L = add(L, mulv(T, bg));
-            } else {
-                const uint4 ra = philox(make_uint4(pixel, sample, depth, 0u), p.key0, p.key1);
-                const uint32_t minfo = sc.info[hid];
-                const Mat m = load_mat(sc.mats[minfo & 0x7FFFFFFFu]);
-                const V3 pos = add(o, muls(d, ht));
-                const V3 N = shading_normal(sc, hid, hu, hv);
-                V3 ff = N;
-                float eta;
-                if (dot(d, ff) > 0.0f) {
-                    ff = neg(ff);
-                    eta = m.ior;
-                    absn = v3(0.0f);
+            if constexpr (BATCH) {
+                uint32_t si;
+                if (FIRST) {
+                    si = find_range(p.pix_off, p.nscenes, p.K, slot);
+                    p.sid[slot] = si;
                 } else {
-                    eta = 1.0f / m.ior;
+                    si = p.sid[slot];
                 }
-                const float area = sc.geo[hid].w;
-                const float pdf_e = emission_pdf(pos, o, neg(d), N, area);
-                V3 e_term = depth == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
-                L = add(L, mulv(T, e_term));
-                if (term) {
-                    // the cache stands for everything after this vertex's emission term
-                    push_query = true;
-                    qf[0] = pos.x; qf[1] = pos.y; qf[2] = pos.z;
-                    qf[3] = -d.x; qf[4] = -d.y; qf[5] = -d.z;
-                    qf[6] = ff.x; qf[7] = ff.y; qf[8] = ff.z;
-                    qf[9] = m.roughness;
-                    qf[10] = m.base.x; qf[11] = m.base.y; qf[12] = m.base.z;
-                    p.thr[slot] = make_float4(T.x, T.y, T.z, 0.0f);
-                }
-                const float len = length(sub(pos, o));
-                const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
-                // DirectLight (src/renderer.cpp:335-384)
-                if (sc.num_emit > 0 && !term) {
-                    const EmitterRec &E = sc.emit[pick(ra.x, sc.num_emit)];
-                    const float sqr1 = sqrtf(u01(ra.y));
-                    const float r2 = u01(ra.z);
-                    const float ea = 1.0f - sqr1, eb = (1.0f - r2) * sqr1;
-                    const V3 epos = add(add(muls(ld3(E.e1), ea), muls(ld3(E.e2), eb)), ld3(E.v0));
-                    const V3 dir_to = normalized(sub(epos, pos));
-                    float te = 0.0f, ue = 0.0f, ve = 0.0f, tmax = kFloatInfinity;
-                    if (E.tri != hid && mt_intersect(pos, dir_to, ld3(E.v0), ld3(E.e1), ld3(E.e2), te, ue, ve) &&
-                        te < kFloatInfinity) {
-                        tmax = te;
-                    } else {
-                        ue = 0.0f;
-                        ve = 0.0f;
-                    }
-                    const bool esmooth = (sc.info[E.tri] & 0x80000000u) != 0u;
-                    const V3 en = esmooth ? shading_normal(sc, E.tri, ue, ve) : ld3(E.normal);
-                    const float lightpdf = emission_pdf(epos, pos, neg(dir_to), en, E.area);
-                    float bsdfpdf;
-                    const V3 f = BsdfEval(m, eta, neg(d), ff, dir_to, bsdfpdf);
-                    const float w = PowerHeuristic(lightpdf, bsdfpdf);
-                    if (bsdfpdf > 0.0f) {
-                        const V3 em = ld3(sc.mats[sc.info[E.tri] & 0x7FFFFFFFu].emission);
-                        const V3 c = divs(muls(mulv(muls(f, w), em), (float)sc.num_emit), lightpdf);
-                        const V3 contrib = mulv(T, mulv(c, br));
-                        if (!(contrib.x == 0.0f && contrib.y == 0.0f && contrib.z == 0.0f)) {
-                            push_shadow = true;
-                            p.sh_d[slot] = make_float4(dir_to.x, dir_to.y, dir_to.z, tmax);
-                            p.sh_c[slot] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(E.tri));
-                        }
-                    }
-                }
-                V3 nd = v3(0.0f);
-                float bpdf = 0.0f;
-                if (!last) {
-                    const uint4 rb = philox(make_uint4(pixel, sample, depth, 1u), p.key0, p.key1);
-                    V3 color;
-                    if (BsdfSample(m, eta, ff, neg(d), u01(ra.w), u01(rb.x), nd, color, bpdf)) {
-                        absn = divs(neg(V3{logf(m.extinction.x), logf(m.extinction.y), logf(m.extinction.z)}),
-                                    m.at_distance);
-                        if (bpdf > 0.0f) {
-                            T = mulv(T, divs(mulv(br, color), bpdf));
-                            push_next = true;
-                        }
-                    }
-                }
-                if (push_next || push_shadow) {
-                    p.ray_o[slot] = make_float4(pos.x, pos.y, pos.z, bpdf);
-                    p.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(hid));
-                }
-                if (push_next) {
-                    p.thr[slot] = make_float4(T.x, T.y, T.z, 0.0f);
-                    p.absn[slot] = make_float4(absn.x, absn.y, absn.z, 0.0f);
-                }
+                const BatchScene &b = p.bscn[si];
+                const uint32_t l = slot - p.K * p.pix_off[si];
+                const uint32_t px = l % b.npix;
+                shade_path<FIRST>(b.sc, b.cam, b.divW, b.divH, p.pix_off[si] + px, px, s0 + l / b.npix, p, slot,
+                                  depth, term, last, push_next, push_shadow, push_query, qf);
+            } else {
+                const uint32_t px = slot % p.npix;
+                shade_path<FIRST>(sc, p.cam, p.divW, p.divH, px, px, s0 + slot / p.npix, p, slot, depth, term, last,
+                                  push_next, push_shadow, push_query, qf);
             }
-            p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
         }
         uint32_t idx[3];
         block_append3(s_app, it, next_ctr, push_next && next_ctr, sh_ctr, push_shadow, q_ctr, push_query, idx);
         if (push_next && next_ctr) q_out[idx[0]] = slot;
         if (push_shadow) p.sq[idx[1]] = slot;
-        {
-            if (push_query) {
-                float *dst = p.cache_q + kNrcFloats * (size_t)idx[2];
-                for (int c = 0; c < 13; ++c) dst[c] = qf[c];
-                dst[kRecBounces] = (float)(p.max_depth - 1u - depth);
-                dst[kRecBounces + 1] = 0.0f;
-                dst[kRecSlot] = __uint_as_float(slot);
-            }
+        if (push_query) {
+            float *dst = p.cache_q + kNrcFloats * (size_t)idx[2];
+            for (int c = 0; c < 13; ++c) dst[c] = qf[c];
+            dst[kRecBounces] = (float)(p.max_depth - 1u - depth);
+            dst[kRecBounces + 1] = 0.0f;
+            dst[kRecSlot] = __uint_as_float(slot);
         }
     }
 }
 
-template <bool SHADOW>
+template <bool SHADOW, bool BATCH>
 __global__ __launch_bounds__(256) void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t depth,
                                                        uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
     __shared__ uint32_t s_ref[kLdsStack * 256];
@@ -265,6 +321,7 @@ __global__ __launch_bounds__(256) void pt_trace_kernel(DevScene sc, PtDev p, uin
         const uint32_t j = chunk * 64u + lane;
         if (j >= count) continue;
         const uint32_t slot = q[j];
+        const DevScene &S = BATCH ? p.bscn[p.sid[slot]].sc : sc;
         const float4 ro = p.ray_o[slot];
         const float4 rd = p.ray_d[slot];
         const V3 o = {ro.x, ro.y, ro.z};
@@ -273,26 +330,34 @@ __global__ __launch_bounds__(256) void pt_trace_kernel(DevScene sc, PtDev p, uin
         if (SHADOW) {
             const float4 sd = p.sh_d[slot];
             const float4 c = p.sh_c[slot];
-            const bool occluded = trace<true, false>(sc, o, V3{sd.x, sd.y, sd.z}, ex, __float_as_uint(c.w), sd.w,
+            const bool occluded = trace<true, false>(S, o, V3{sd.x, sd.y, sd.z}, ex, __float_as_uint(c.w), sd.w,
                                                      stk, h, nn, nt);
             if (!occluded) {
                 float4 l = p.rad[slot];
                 p.rad[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
             }
         } else {
-            trace<false, false>(sc, o, V3{rd.x, rd.y, rd.z}, ex, kNone, 0.0f, stk, h, nn, nt);
+            trace<false, false>(S, o, V3{rd.x, rd.y, rd.z}, ex, kNone, 0.0f, stk, h, nn, nt);
             p.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
         }
     }
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(256) void pt_accum_kernel(PtDev p, uint32_t K) {
     const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
     if (px >= p.npix) return;
     const float inv = (float)p.spp;
+    // slots of pixel px for samples k = 0..K-1: base + k * stride
+    size_t base = px, stride = p.npix;
+    if constexpr (BATCH) {
+        const uint32_t si = find_range(p.pix_off, p.nscenes, 1u, px);
+        stride = p.bscn[si].npix;
+        base = (size_t)K * p.pix_off[si] + (px - p.pix_off[si]);
+    }
     V3 acc = {p.fb[3 * (size_t)px], p.fb[3 * (size_t)px + 1], p.fb[3 * (size_t)px + 2]};
     for (uint32_t k = 0; k < K; ++k) {
-        const float4 l = p.rad[(size_t)k * p.npix + px];
+        const float4 l = p.rad[base + (size_t)k * stride];
         acc = add(acc, divs(V3{l.x, l.y, l.z}, inv));
     }
     p.fb[3 * (size_t)px] = acc.x;
@@ -311,11 +376,11 @@ __global__ void pt_zero_u32_kernel(uint32_t *p, size_t n) {
 
 int pt_trace_blocks_per_cu() {
     int nb = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false, false>, 256, 0);
     int nb2 = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true, true>, 256, 0);
     int nb3 = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb3, pt_primary_kernel, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb3, pt_primary_kernel<true>, 256, 0);
     nb = nb < nb2 ? nb : nb2;
     nb = nb < nb3 ? nb : nb3;
     return nb > 0 ? nb : 1;
@@ -332,41 +397,64 @@ hipError_t pt_enqueue_frame_start(const PtDev &p, size_t ctr_words, hipStream_t 
 
 hipError_t pt_enqueue_primary(const DevScene &sc, const PtDev &p, uint32_t grid, uint32_t *ovf_ref, float *ovf_t,
                               int64_t ovf_stride, hipStream_t s) {
-    const uint32_t tiles_x = (p.cam.width + 7) / 8, tiles_y = (p.cam.height + 7) / 8;
-    hipLaunchKernelGGL(pt_primary_kernel, dim3(grid), dim3(256), 0, s, sc, p, tiles_x, tiles_x * tiles_y, ovf_ref,
-                       ovf_t, ovf_stride);
+    if (p.nscenes) {
+        hipLaunchKernelGGL(pt_primary_kernel<true>, dim3(grid), dim3(256), 0, s, sc, p, 0u, p.ntiles, ovf_ref, ovf_t,
+                           ovf_stride);
+    } else {
+        const uint32_t tiles_x = (p.cam.width + 7) / 8, tiles_y = (p.cam.height + 7) / 8;
+        hipLaunchKernelGGL(pt_primary_kernel<false>, dim3(grid), dim3(256), 0, s, sc, p, tiles_x, tiles_x * tiles_y,
+                           ovf_ref, ovf_t, ovf_stride);
+    }
     return hipGetLastError();
+}
+
+template <bool BATCH>
+static void enqueue_bounce_loop(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t s0, uint32_t npaths,
+                                uint32_t depth_end, uint32_t trace_grid, uint32_t shade_grid, uint32_t *ovf_ref,
+                                float *ovf_t, int64_t ovf_stride, bool shadows, const PtCache &cache, hipStream_t s,
+                                hipError_t &err) {
+    for (uint32_t d = 0; d < depth_end; ++d) {
+        if (d == 0) {
+            hipLaunchKernelGGL((pt_shade_kernel<true, BATCH>), dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0,
+                               npaths, 0u);
+        } else {
+            hipLaunchKernelGGL((pt_trace_kernel<false, BATCH>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d,
+                               ovf_ref, ovf_t, ovf_stride);
+            hipLaunchKernelGGL((pt_shade_kernel<false, BATCH>), dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0,
+                               npaths, d);
+        }
+        if (p.cache_depth && d == p.cache_depth) {
+            // the queries of this depth were counted in its shadow-fetch counter (no shadow rays here)
+            err = nrc_launch_infer(*cache.grid, cache.w16, p.cache_q, ctr_at_host(p, batch, d, 3), 0, nullptr, p.rad,
+                                   p.thr, cache.infer_grid, 1, s);
+            if (err != hipSuccess) return;
+            break;
+        }
+        if (shadows)
+            hipLaunchKernelGGL((pt_trace_kernel<true, BATCH>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d,
+                               ovf_ref, ovf_t, ovf_stride);
+    }
 }
 
 hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t s0, uint32_t K,
                             uint32_t trace_grid, uint32_t shade_grid, uint32_t *ovf_ref, float *ovf_t,
-                            int64_t ovf_stride, const PtCache &cache, hipStream_t s) {
+                            int64_t ovf_stride, bool shadows, const PtCache &cache, hipStream_t s) {
     const uint32_t npaths = K * p.npix;
     const uint32_t depth_end = p.cache_depth ? p.cache_depth + 1 : p.max_depth;
     hipLaunchKernelGGL(pt_zero_u32_kernel, dim3((unsigned)((kFetchWords + 255) / 256)), dim3(256), 0, s, p.fctr,
                        kFetchWords);
-    for (uint32_t d = 0; d < depth_end; ++d) {
-        if (d == 0) {
-            hipLaunchKernelGGL(pt_shade_kernel<true>, dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0, npaths,
-                               0u);
-        } else {
-            hipLaunchKernelGGL(pt_trace_kernel<false>, dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d, ovf_ref,
-                               ovf_t, ovf_stride);
-            hipLaunchKernelGGL(pt_shade_kernel<false>, dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0, npaths,
-                               d);
-        }
-        if (p.cache_depth && d == p.cache_depth) {
-            // the queries of this depth were counted in its shadow-fetch counter (no shadow rays here)
-            hipError_t e = nrc_launch_infer(*cache.grid, cache.w16, p.cache_q, ctr_at_host(p, batch, d, 3), 0, nullptr,
-                                            p.rad, p.thr, cache.infer_grid, 1, s);
-            if (e != hipSuccess) return e;
-            break;
-        }
-        if (sc.num_emit > 0)
-            hipLaunchKernelGGL(pt_trace_kernel<true>, dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d, ovf_ref,
-                               ovf_t, ovf_stride);
-    }
-    hipLaunchKernelGGL(pt_accum_kernel, dim3((p.npix + 255) / 256), dim3(256), 0, s, p, K);
+    hipError_t err = hipSuccess;
+    if (p.nscenes)
+        enqueue_bounce_loop<true>(sc, p, batch, s0, npaths, depth_end, trace_grid, shade_grid, ovf_ref, ovf_t,
+                                  ovf_stride, shadows, cache, s, err);
+    else
+        enqueue_bounce_loop<false>(sc, p, batch, s0, npaths, depth_end, trace_grid, shade_grid, ovf_ref, ovf_t,
+                                   ovf_stride, shadows, cache, s, err);
+    if (err != hipSuccess) return err;
+    if (p.nscenes)
+        hipLaunchKernelGGL(pt_accum_kernel<true>, dim3((p.npix + 255) / 256), dim3(256), 0, s, p, K);
+    else
+        hipLaunchKernelGGL(pt_accum_kernel<false>, dim3((p.npix + 255) / 256), dim3(256), 0, s, p, K);
     return hipGetLastError();
 }
 

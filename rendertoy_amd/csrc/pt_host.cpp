@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "nrc.h"
 #include "pt.h"
 #include "pt_kernels.h"
@@ -76,6 +77,19 @@ void free_all(PtState *st) {
 
 }  // namespace
 
+void detach_batches(rt_scene *scene) {
+    for (rt_batch *b : scene->batches) {
+        b->detached = true;
+        for (rt_scene *m : b->scenes)
+            if (m != scene) {
+                auto &v = m->batches;
+                v.erase(std::remove(v.begin(), v.end(), b), v.end());
+            }
+        b->scenes.clear();
+    }
+    scene->batches.clear();
+}
+
 void pt_destroy(PtState *p) {
     if (!p) return;
     free_all(p);
@@ -96,14 +110,18 @@ int pt_device_output(rt_scene *s, void **dev_ptr, size_t *bytes) {
 
 using namespace rtamd;
 
-static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params *prm, float *rgb_host,
-                          rt_cache *cache) {
+// One path-traced frame on the render context s (stream, events, traversal
+// overflow region, path state, statistics): of the scene s itself (batch null,
+// camera cam, result to rgb_host), or of every scene of a batch (cameras in
+// batch->h_table, results to rgb_hosts[i]).
+static int render_frame(rt_scene *s, const rt_camera *cam, rt_batch *batch, const rt_pt_params *prm,
+                        float *rgb_host, float *const *rgb_hosts, rt_cache *cache) {
     try {
-        if (!s || !cam || !prm) {
+        if (!s || (!cam && !batch) || !prm) {
             set_error("scene, camera or params is NULL");
             return RT_ERR_INVALID;
         }
-        if (cam->width == 0 || cam->height == 0 || (uint64_t)cam->width * cam->height >= (1ull << 31)) {
+        if (cam && (cam->width == 0 || cam->height == 0 || (uint64_t)cam->width * cam->height >= (1ull << 31))) {
             set_error("bad frame size");
             return RT_ERR_INVALID;
         }
@@ -114,9 +132,17 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
         hipchk(hipSetDevice(s->device), "hipSetDevice");
         if (!s->pt) s->pt = new PtState();
         PtState *st = s->pt;
-        const uint32_t npix = cam->width * cam->height;
-        // batch size: about 4M paths in flight unless the caller says otherwise
-        uint64_t target = prm->batch_paths ? prm->batch_paths : (4u << 20);
+        const uint32_t npix = batch ? batch->total_pix : cam->width * cam->height;
+        // batch size: up to 64M paths in flight (about 150 B of state each), at most
+        // an eighth of the free HBM, unless the caller says otherwise.  Fewer,
+        // larger batches cut the bounce-loop launches and their tails.
+        uint64_t target = prm->batch_paths;
+        if (!target) {
+            size_t free_b = 0, total_b = 0;
+            hipchk(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+            const uint64_t per_path = 8 * 16 + 3 * 4 + (cache ? kNrcFloats * 4 : 0);
+            target = std::min<uint64_t>(64u << 20, std::max<uint64_t>(1u << 20, free_b / 8 / per_path));
+        }
         uint32_t K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(prm->spp, target / npix));
         uint32_t nb = (prm->spp + K - 1) / K;
         if (nb > kMaxBatches) {
@@ -129,7 +155,7 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
             return RT_ERR_INVALID;
         }
         const size_t ctr_words = (size_t)nb * kMaxDepth * 4 * 32;
-        if (st->cap < cap || st->npix != npix || st->ctr_words < ctr_words) {
+        if (st->cap < cap || st->npix != npix || st->ctr_words < ctr_words || (batch && !st->dev.sid)) {
             free_all(st);
             PtDev &p = st->dev;
             p.ray_o = (float4 *)alloc(st, cap * 16);
@@ -148,6 +174,7 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
             p.ctr = (uint32_t *)alloc(st, ctr_words * 4);
             p.prim_ctr = (uint32_t *)alloc(st, 256 * 4);
             p.fctr = (uint32_t *)alloc(st, kFetchWords * 4);
+            p.sid = batch ? (uint32_t *)alloc(st, cap * 4) : nullptr;
             st->cap = cap;
             st->npix = npix;
             st->ctr_words = ctr_words;
@@ -171,13 +198,33 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
             if (st->tune_trial < kNumTraceShares) trial = st->tune_trial;
             p.share_trace = kTraceShares[trial < kNumTraceShares ? trial : st->tune_best];
         }
-        p.cam = *cam;
+        if (batch) {
+            std::memset(&p.cam, 0, sizeof(p.cam));
+            p.divW = p.divH = 0.0f;
+            p.nscenes = (uint32_t)batch->scenes.size();
+            p.bscn = batch->d_table;
+            p.pix_off = batch->d_off;
+            p.tile_off = batch->d_off + p.nscenes + 1;
+            p.K = K;
+            p.ntiles = batch->total_tiles;
+            // cameras may change from frame to frame: the table is read on the device
+            hipchk(hipMemcpyAsync(batch->d_table, batch->h_table.data(), batch->h_table.size() * sizeof(BatchScene),
+                                  hipMemcpyHostToDevice, s->stream),
+                   "hipMemcpyAsync");
+        } else {
+            p.cam = *cam;
+            p.divW = 1.0f / float(cam->width);
+            p.divH = 1.0f / float(cam->height);
+            p.nscenes = 0;
+            p.bscn = nullptr;
+            p.pix_off = p.tile_off = nullptr;
+            p.K = K;
+            p.ntiles = 0;
+        }
         p.cache_depth = cache ? cache->prm.first_vertex : 0;
         p.cache_q = cache ? st->cache_q : nullptr;
         PtCache pc{cache ? &cache->grid : nullptr, cache ? cache->tr.w16 : nullptr,
                    (uint32_t)s->cu_count * 4};
-        p.divW = 1.0f / float(cam->width);
-        p.divH = 1.0f / float(cam->height);
         p.npix = npix;
         p.spp = prm->spp;
         p.max_depth = prm->max_depth;
@@ -188,6 +235,7 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
         ensure_overflow(s, trace_grid * 256);
         const int64_t ovf_stride = (int64_t)trace_grid * 256;
         const uint32_t shade_grid = (uint32_t)std::min<uint64_t>((cap + 255) / 256, (uint64_t)s->cu_count * 8);
+        const bool shadows = batch ? batch->any_emitters : s->ds.num_emit > 0;
 
         auto enqueue = [&](hipStream_t strm) {
             hipchk(pt_enqueue_frame_start(p, ctr_words, strm), "frame start");
@@ -196,17 +244,17 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
                 uint32_t s0 = b * K;
                 uint32_t Kb = std::min(K, prm->spp - s0);
                 hipchk(pt_enqueue_batch(s->ds, p, b, s0, Kb, trace_grid, shade_grid, s->d_ovf_ref, s->d_ovf_t,
-                                        ovf_stride, pc, strm),
+                                        ovf_stride, shadows, pc, strm),
                        "batch");
             }
         };
 
         // graph key: everything the captured launches depend on
         // (a cache enters through its buffers, which a new cache at a reused address does not share)
-        uint64_t extra[10] = {nb, K, trace_grid, shade_grid, (uint64_t)(uintptr_t)s->d_ovf_ref,
+        uint64_t extra[11] = {nb, K, trace_grid, shade_grid, (uint64_t)(uintptr_t)s->d_ovf_ref,
                               (uint64_t)(uintptr_t)s->d_ovf_t, (uint64_t)ovf_stride, ctr_words,
                               (uint64_t)(uintptr_t)(cache ? cache->grid.t16 : nullptr),
-                              (uint64_t)(uintptr_t)(cache ? cache->tr.w16 : nullptr)};
+                              (uint64_t)(uintptr_t)(cache ? cache->tr.w16 : nullptr), shadows ? 1u : 0u};
         std::vector<uint8_t> key(sizeof(PtDev) + sizeof(extra));
         std::memcpy(key.data(), &p, sizeof(PtDev));
         std::memcpy(key.data() + sizeof(PtDev), extra, sizeof(extra));
@@ -275,7 +323,14 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
         sts.samples = (uint64_t)npix * prm->spp;
         sts.cache_queries = queries;
         sts.node_visits = sts.tri_tests = 0;
-        if (rgb_host) hipchk(hipMemcpy(rgb_host, p.fb, (size_t)npix * 12, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (!batch && rgb_host)
+            hipchk(hipMemcpy(rgb_host, p.fb, (size_t)npix * 12, hipMemcpyDeviceToHost), "hipMemcpy");
+        if (batch && rgb_hosts)
+            for (size_t i = 0; i < batch->scenes.size(); ++i)
+                if (rgb_hosts[i])
+                    hipchk(hipMemcpy(rgb_hosts[i], p.fb + 3 * (size_t)batch->h_off[i],
+                                     (size_t)batch->h_table[i].npix * 12, hipMemcpyDeviceToHost),
+                           "hipMemcpy");
         return RT_OK;
     } catch (const std::bad_alloc &) {
         set_error("host out of memory");
@@ -287,7 +342,11 @@ static int render_pt_impl(rt_scene *s, const rt_camera *cam, const rt_pt_params 
 }
 
 extern "C" int rt_render_pt(rt_scene *s, const rt_camera *cam, const rt_pt_params *prm, float *rgb_host) {
-    return render_pt_impl(s, cam, prm, rgb_host, nullptr);
+    if (!cam) {
+        set_error("camera is NULL");
+        return RT_ERR_INVALID;
+    }
+    return render_frame(s, cam, nullptr, prm, rgb_host, nullptr, nullptr);
 }
 
 extern "C" int rt_render_pt_cached(rt_scene *s, rt_cache *cache, const rt_camera *cam, const rt_pt_params *prm,
@@ -296,6 +355,131 @@ extern "C" int rt_render_pt_cached(rt_scene *s, rt_cache *cache, const rt_camera
         set_error("cache is NULL or belongs to another scene");
         return RT_ERR_INVALID;
     }
-    if (cache->prm.first_vertex >= prm->max_depth) return render_pt_impl(s, cam, prm, rgb_host, nullptr);
-    return render_pt_impl(s, cam, prm, rgb_host, cache);
+    if (!cam || !prm) {
+        set_error("camera or params is NULL");
+        return RT_ERR_INVALID;
+    }
+    if (cache->prm.first_vertex >= prm->max_depth) return render_frame(s, cam, nullptr, prm, rgb_host, nullptr, nullptr);
+    return render_frame(s, cam, nullptr, prm, rgb_host, nullptr, cache);
 }
+
+rt_batch::~rt_batch() {
+    if (ctx) (void)hipStreamSynchronize(ctx->stream);
+    if (d_table) (void)hipFree(d_table);
+    if (d_off) (void)hipFree(d_off);
+    delete ctx;
+}
+
+extern "C" {
+
+int rt_batch_create(rt_scene *const *scenes, uint32_t n, rt_batch **out) {
+    if (!out) {
+        set_error("out is NULL");
+        return RT_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (!scenes || n == 0 || n > kMaxBatchScenes) {
+        set_error("a batch holds 1 to 4096 scenes");
+        return RT_ERR_INVALID;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (!scenes[i] || scenes[i]->device != scenes[0]->device) {
+            set_error("batch scenes must be non-NULL and on one device");
+            return RT_ERR_INVALID;
+        }
+    rt_batch *b = new (std::nothrow) rt_batch();
+    if (!b) {
+        set_error("host out of memory");
+        return RT_ERR_OOM;
+    }
+    try {
+        b->ctx = make_render_context(scenes[0]->device);
+        b->scenes.assign(scenes, scenes + n);
+        b->h_table.resize(n);
+        b->h_off.assign(2 * ((size_t)n + 1), 0u);
+        for (uint32_t i = 0; i < n; ++i) {
+            std::memset(&b->h_table[i], 0, sizeof(BatchScene));
+            b->h_table[i].sc = scenes[i]->ds;
+            b->any_emitters |= scenes[i]->ds.num_emit > 0;
+            b->ctx->max_stack = std::max(b->ctx->max_stack, scenes[i]->max_stack);
+        }
+        hipchk(hipMalloc(&b->d_table, (size_t)n * sizeof(BatchScene)), "hipMalloc");
+        hipchk(hipMalloc(&b->d_off, b->h_off.size() * sizeof(uint32_t)), "hipMalloc");
+        for (uint32_t i = 0; i < n; ++i) scenes[i]->batches.push_back(b);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        delete b;
+        return RT_ERR_HIP;
+    }
+    *out = b;
+    return RT_OK;
+}
+
+int rt_batch_destroy(rt_batch *b) {
+    if (!b) return RT_OK;
+    for (rt_scene *m : b->scenes) {
+        auto &v = m->batches;
+        v.erase(std::remove(v.begin(), v.end(), b), v.end());
+    }
+    delete b;
+    return RT_OK;
+}
+
+int rt_batch_render(rt_batch *b, const rt_camera *cams, const rt_pt_params *prm, float *const *rgb_host) {
+    if (!b || !cams || !prm) {
+        set_error("batch, cameras and params are required");
+        return RT_ERR_INVALID;
+    }
+    if (b->detached) {
+        set_error("a scene of this batch was destroyed");
+        return RT_ERR_INVALID;
+    }
+    const uint32_t n = (uint32_t)b->scenes.size();
+    uint64_t pix = 0, tiles = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const rt_camera &c = cams[i];
+        if (c.width == 0 || c.height == 0) {
+            set_error("batch camera with zero width or height");
+            return RT_ERR_INVALID;
+        }
+        BatchScene &t = b->h_table[i];
+        t.cam = c;
+        t.divW = 1.0f / float(c.width);
+        t.divH = 1.0f / float(c.height);
+        t.npix = c.width * c.height;
+        t.tiles_x = (c.width + 7) / 8;
+        b->h_off[i] = (uint32_t)pix;
+        b->h_off[n + 1 + i] = (uint32_t)tiles;
+        pix += (uint64_t)c.width * c.height;
+        tiles += (uint64_t)t.tiles_x * ((c.height + 7) / 8);
+        if (pix >= (1ull << 31)) {
+            set_error("batch frames hold more than 2^31 pixels");
+            return RT_ERR_INVALID;
+        }
+    }
+    b->h_off[n] = (uint32_t)pix;
+    b->h_off[2 * n + 1] = (uint32_t)tiles;
+    b->total_pix = (uint32_t)pix;
+    b->total_tiles = (uint32_t)tiles;
+    try {
+        hipchk(hipSetDevice(b->ctx->device), "hipSetDevice");
+        hipchk(hipMemcpyAsync(b->d_off, b->h_off.data(), b->h_off.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              b->ctx->stream),
+               "hipMemcpyAsync");
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return RT_ERR_HIP;
+    }
+    return render_frame(b->ctx, nullptr, b, prm, nullptr, rgb_host, nullptr);
+}
+
+int rt_batch_stats(const rt_batch *b, rt_stats *out) {
+    if (!b || !out) {
+        set_error("batch and out are required");
+        return RT_ERR_INVALID;
+    }
+    *out = b->ctx->stats;
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -11,10 +11,20 @@
 #include <stdint.h>
 
 #include "../../include/rendertoy_amd.h"
+#include "dev_trace.h"
 
 namespace rtamd {
 
-struct DevScene;
+// One scene of a batched render (rt_batch): its geometry and camera.
+struct BatchScene {
+    DevScene sc;
+    rt_camera cam;
+    float divW, divH;
+    uint32_t npix;        // width * height
+    uint32_t tiles_x;     // 8x8 primary tiles per row
+};
+
+constexpr uint32_t kMaxBatchScenes = 4096;
 
 constexpr uint32_t kMaxDepth = 64;      // path vertices
 constexpr uint32_t kMaxBatches = 4096;
@@ -32,6 +42,14 @@ struct PtDev {
     uint32_t *fctr;       // trace-kernel band counters: [kMaxDepth][extend, shadow][8], 32-word stride,
                           // cleared at the start of each batch
     uint32_t share_trace; // trace-queue scheduling policy (BandSched share)
+    // batched multi-scene mode (nscenes > 0): scene table, pixel and primary-tile
+    // offsets of each scene (nscenes + 1 entries), per-slot scene index, samples
+    // per batch; npix is then the pixel total
+    uint32_t nscenes;
+    const BatchScene *bscn;
+    const uint32_t *pix_off, *tile_off;
+    uint32_t *sid;
+    uint32_t K, ntiles;
     float *fb;            // npix * 3
     // radiance cache: paths reaching vertex cache_depth (>= 1; 0 = off) add their
     // emission term there and leave a cache query (16 floats, slot in [13]).
@@ -51,6 +69,6 @@ struct PtCache {
 };
 hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t s0, uint32_t K,
                             uint32_t trace_grid, uint32_t shade_grid, uint32_t *ovf_ref, float *ovf_t,
-                            int64_t ovf_stride, const PtCache &cache, hipStream_t s);
+                            int64_t ovf_stride, bool shadows, const PtCache &cache, hipStream_t s);
 
 }  // namespace rtamd

@@ -51,13 +51,18 @@ struct rt_scene {
     // path tracer state
     rtamd::PtState *pt = nullptr;
     rt_stats stats{};
-    // radiance caches built on this scene (detached when the scene goes first)
+    // radiance caches built on this scene and batches that hold it (both are
+    // detached when the scene goes first)
     std::vector<rt_cache *> caches;
+    std::vector<rt_batch *> batches;
 
     ~rt_scene();
 };
 
 namespace rtamd {
-// Called by rt_scene_destroy: the scene's radiance caches stop referring to it.
+// Called by rt_scene_destroy: the scene's radiance caches and batches stop referring to it.
 void detach_caches(rt_scene *scene);
+void detach_batches(rt_scene *scene);
+// A render context without geometry (stream, events, path state) on a device.
+rt_scene *make_render_context(int device);
 }  // namespace rtamd

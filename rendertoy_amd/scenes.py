@@ -23,6 +23,7 @@ Workloads (sizes fixed by the benchmark definitions):
 """
 from __future__ import annotations
 
+import dataclasses
 import hashlib
 import math
 from dataclasses import dataclass, field
@@ -387,6 +388,24 @@ def terrain_c2(W=1920, H=1080, spp=64, max_depth=8, cells=1024, n_spheres=256, s
     cam = look_camera((-70.0, -10.0, 30.0), 10.0, -22.0, 24.0)
     return _finish("terrain_c2", tris, norms, np.concatenate(tri_mesh), mesh_mat, smooth, mats, cam,
                    W, H, spp, max_depth)
+
+
+def batch_c3(n=64, seed=3, spp=32, max_depth=6, tri_range=(10_000, 500_000), resolutions=(128, 256, 512, 1024)):
+    """A batch of n terrain scenes for batched rendering: triangle counts
+    log-uniform in tri_range (three quarters heightfield, one quarter
+    icospheres of 1,280 triangles, plus the 3 emissive quads), square frames
+    of a resolution drawn from `resolutions`, spp samples, depth max_depth."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        t = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+        cells = max(8, int(np.sqrt(0.75 * t / 2.0)))
+        spheres = max(1, int(round(0.25 * t / 1280.0)))
+        res = int(rng.choice(resolutions))
+        sc = terrain_c2(res, res, spp, max_depth, cells=cells, n_spheres=spheres, n_mats=16,
+                        seed=int(rng.integers(1 << 30)))
+        out.append(dataclasses.replace(sc, name="batch_c3_%02d" % i))
+    return out
 
 
 def material_zoo(W=64, H=64, spp=4, max_depth=5) -> Scene:

@@ -322,11 +322,11 @@ __device__ __forceinline__ h4 gl_b(const _Float16 *tile, int f0, int ks, uint32_
     return *reinterpret_cast<const h4 *>(tile + (f0 + (lane & 15)) * 64 + 16 * ks + 4 * (lane >> 4));
 }
 
-// Backward.  wpart: per wave, kNrcParams fp32 partial weight gradients (scaled).
-// grid_grad: fp32 table gradients (scaled), scatter-added.
+// Backward.  wpart: per wave, kNrcParams fp32 partial weight gradients
+// (scaled); dgrid: per sample, the 32 grid-feature input gradients (scaled).
 __global__ __launch_bounds__(256) void nrc_train_bwd_kernel(NrcGrid g, const _Float16 *wt16, const float *samples,
                                                             uint32_t n, const _Float16 *act, const _Float16 *dout,
-                                                            float *wpart) {
+                                                            float *wpart, float *dgrid) {
     __shared__ _Float16 lds[kLdsWT];
     __shared__ _Float16 dz_tile[4][64 * 64];
     // W_j^T (rows = inputs) into LDS
@@ -434,31 +434,16 @@ __global__ __launch_bounds__(256) void nrc_train_bwd_kernel(NrcGrid g, const _Fl
                             dz[t][nt][i] = (_Float16)(y > (_Float16)0.0f ? dx[t][nt][i] : 0.0f);
                         }
             } else {
-                // input gradient of the grid features (t = 0: levels 2g, 2g+1; t = 1: 8+2g, 9+2g)
+                // input gradient of the 32 grid features (feature 16t + 4g + i), scattered
+                // into the table by nrc_grid_scatter_kernel
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt) {
                     const uint32_t q = w * 64 + 16 * nt + (lane & 15);
                     if (q >= n) continue;
-                    float p[3];
-                    unit_pos(g, samples + kNrcFloats * (size_t)q, p);
 #pragma unroll
                     for (int t = 0; t < 2; ++t)
-#pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            const int l = 8 * t + 2 * grp + h;
-                            const float ga = dx[t][nt][2 * h], gb = dx[t][nt][2 * h + 1];
-                            if (ga == 0.0f && gb == 0.0f) continue;
-                            const Corners c = level_cell(g, l, p);
-                            float *gg = g.grad + (size_t)l * (g.tmask + 1) * 2;
-#pragma unroll
-                            for (int k = 0; k < 8; ++k) {
-                                uint32_t idx;
-                                float wgt;
-                                corner(g, l, c, k, idx, wgt);
-                                atomicAdd(gg + 2 * idx, wgt * ga);
-                                atomicAdd(gg + 2 * idx + 1, wgt * gb);
-                            }
-                        }
+                        *reinterpret_cast<float4 *>(dgrid + 32 * (size_t)q + 16 * t + 4 * grp) =
+                            make_float4(dx[t][nt][0], dx[t][nt][1], dx[t][nt][2], dx[t][nt][3]);
                 }
             }
         }
@@ -466,7 +451,46 @@ __global__ __launch_bounds__(256) void nrc_train_bwd_kernel(NrcGrid g, const _Fl
     }
 }
 
-// Adam on the MLP: reduce the per-wave partials in wave order (deterministic).
+// Grid gradient: one thread per (sample, level) adds the two features'
+// input gradients, times the trilinear weights, to the level's 8 corners.  A
+// wave takes one level of 64 consecutive samples (spatially coherent when the
+// samples are), so its atomics land on nearby entries.
+__global__ __launch_bounds__(256) void nrc_grid_scatter_kernel(NrcGrid g, const float *samples, const float *dgrid,
+                                                               uint32_t n) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wave = t >> 6;
+    const uint32_t l = wave % kNrcLevels, q = (wave / kNrcLevels) * 64u + (t & 63u);
+    if (q >= n) return;
+    // level l's two features are inputs 2l and 2l + 1
+    const float2 gd = *reinterpret_cast<const float2 *>(dgrid + 32 * (size_t)q + 2 * l);
+    if (gd.x == 0.0f && gd.y == 0.0f) return;
+    float p[3];
+    unit_pos(g, samples + kNrcFloats * (size_t)q, p);
+    const Corners c = level_cell(g, (int)l, p);
+    float *gg = g.grad + (size_t)l * (g.tmask + 1) * 2;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        uint32_t idx;
+        float wgt;
+        corner(g, (int)l, c, k, idx, wgt);
+        atomicAdd(gg + 2 * idx, wgt * gd.x);
+        atomicAdd(gg + 2 * idx + 1, wgt * gd.y);
+    }
+}
+
+// First pass of the MLP-gradient reduction: group k of kRedGroups sums the
+// per-wave partials of waves k, k + kRedGroups, ... in order (deterministic).
+constexpr uint32_t kRedGroups = 32;
+__global__ void nrc_reduce_mlp_kernel(const float *wpart, uint32_t nwaves, float *red) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = blockIdx.y;
+    if (p >= (uint32_t)kNrcParams) return;
+    float s = 0.0f;
+    for (uint32_t w = k; w < nwaves; w += kRedGroups) s += wpart[(size_t)w * kNrcParams + p];
+    red[(size_t)k * kNrcParams + p] = s;
+}
+
+// Adam on the MLP over the reduced gradient (second pass: groups in order).
 __global__ void nrc_adam_mlp_kernel(float *w, float *m, float *v, _Float16 *w16, _Float16 *wt16, const float *wpart,
                                     uint32_t nwaves, float inv_scale, float lr, float b1, float b2, float eps,
                                     float c1, float c2) {
@@ -481,7 +505,7 @@ __global__ void nrc_adam_mlp_kernel(float *w, float *m, float *v, _Float16 *w16,
         return;
     }
     float gsum = 0.0f;
-    for (uint32_t k = 0; k < nwaves; ++k) gsum += wpart[(size_t)k * kNrcParams + p];
+    for (uint32_t k = 0; k < nwaves; ++k) gsum += wpart[(size_t)k * kNrcParams + p];   // nwaves = groups here
     const float gr = gsum * inv_scale;
     const float mm = b1 * m[p] + (1.0f - b1) * gr;
     const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
@@ -696,7 +720,9 @@ hipError_t nrc_launch_train_step(const NrcGrid &g, NrcTrain &t, const float *sam
     hipLaunchKernelGGL(nrc_train_fwd_kernel, dim3(blocks), dim3(256), 0, s, g, t.w16, samples, n, t.act, t.dout,
                        t.loss_part, t.pred_out);
     hipLaunchKernelGGL(nrc_train_bwd_kernel, dim3(blocks), dim3(256), 0, s, g, t.wt16, samples, n, t.act, t.dout,
-                       t.wpart);
+                       t.wpart, t.dgrid);
+    const uint32_t threads = waves * 64u * (uint32_t)kNrcLevels;
+    hipLaunchKernelGGL(nrc_grid_scatter_kernel, dim3((threads + 255) / 256), dim3(256), 0, s, g, samples, t.dgrid, n);
     return hipGetLastError();
 }
 
@@ -706,8 +732,10 @@ hipError_t nrc_launch_adam(const NrcGrid &g, NrcTrain &t, uint32_t n, int step, 
     const float c1 = 1.0f / (1.0f - powf(b1, (float)step)), c2 = 1.0f / (1.0f - powf(b2, (float)step));
     const float inv_scale = 1.0f / (3.0f * (float)n);
     const uint32_t waves = (n + 63) / 64;
+    hipLaunchKernelGGL(nrc_reduce_mlp_kernel, dim3((kNrcParams + 255) / 256, kRedGroups), dim3(256), 0, s, t.wpart,
+                       waves, t.red);
     hipLaunchKernelGGL(nrc_adam_mlp_kernel, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, t.w, t.m, t.v, t.w16,
-                       t.wt16, t.wpart, waves, inv_scale, lr_mlp, b1, b2, eps, c1, c2);
+                       t.wt16, t.red, kRedGroups, inv_scale, lr_mlp, b1, b2, eps, c1, c2);
     if (update_grid) {
         const size_t ng = g.grid_params;
         hipLaunchKernelGGL(nrc_adam_grid_kernel, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, t.grid,

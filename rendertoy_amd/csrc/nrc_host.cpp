@@ -38,6 +38,8 @@ void ensure_batch(rt_cache *c, uint32_t n) {
     t.dout = (_Float16 *)alloc(c, (size_t)waves * 16 * 64 * sizeof(_Float16));
     t.loss_part = (float *)alloc(c, (size_t)waves * sizeof(float));
     t.wpart = (float *)alloc(c, (size_t)waves * kNrcParams * sizeof(float));
+    t.dgrid = (float *)alloc(c, (size_t)n * 32 * sizeof(float));
+    if (!t.red) t.red = (float *)alloc(c, (size_t)32 * kNrcParams * sizeof(float));
     t.pred_out = (float *)alloc(c, (size_t)n * 4 * sizeof(float));
     c->capacity = n;
 }

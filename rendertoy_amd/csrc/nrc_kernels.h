@@ -41,6 +41,8 @@ struct NrcTrain {
     _Float16 *act, *dout;       // per-wave activations / loss gradient
     float *loss_part;           // per-wave loss sums
     float *wpart;               // per-wave weight-gradient partials
+    float *red;                 // weight gradient reduced in 32 groups of waves
+    float *dgrid;               // per-sample input gradients of the 32 grid features
     float *pred_out;            // optional: predictions of the last forward
 };
 

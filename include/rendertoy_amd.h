@@ -140,7 +140,6 @@ typedef struct rt_pt_params {
 
 enum rt_pt_flags {
     RT_PT_NO_GRAPH = 1u << 0,    /* launch the bounce loop kernel by kernel instead of replaying a hipGraph */
-    RT_PT_NO_SORT = 1u << 1,     /* keep extension rays in path order (no coherence sort) */
     RT_PT_NO_TUNE = 1u << 2      /* fixed trace-queue policy; by default the first renders of a new
                                     frame shape try each policy once and keep the fastest */
 };
@@ -254,6 +253,48 @@ int rt_batch_render(rt_batch *batch, const rt_camera *cams, const rt_pt_params *
 
 /* Counters and timings of the batch's last frame (all scenes together). */
 int rt_batch_stats(const rt_batch *batch, rt_stats *out);
+
+/* ---------------------------------------------------------------------------
+ * Post-processing on the device: RenderToy's Image (include/RenderToy/
+ * compositor.h:17-292, src/compositor.cpp).  An rt_image is float[height]
+ * [width][3] in device memory, row-major like RenderContext::buffer.  Results
+ * equal the reference's bit for bit (powf in rt_image_gamma aside, within
+ * 2 ulp) wherever the reference reads inside its buffer; the reference's
+ * reflect edge mode reads up to a row past the buffer on the last rows and
+ * columns (undefined behaviour there), where these functions read black.
+ * Calls are synchronous.
+ * ------------------------------------------------------------------------- */
+typedef struct rt_image rt_image;
+
+/* Convolution flags: Image::Convolute<_Normalize, _Length, _Abs> (compositor.h:107). */
+enum rt_conv_flags { RT_CONV_NORMALIZE = 1u << 0, RT_CONV_LENGTH = 1u << 1, RT_CONV_ABS = 1u << 2 };
+
+/* Image(const SizeN &) (compositor.cpp:6-10): a black image on the current device. */
+int rt_image_create(uint32_t width, uint32_t height, rt_image **out);
+int rt_image_destroy(rt_image *img);
+int rt_image_upload(rt_image *img, const float *rgb_host);
+int rt_image_download(const rt_image *img, float *rgb_host);
+int rt_image_device_buffer(rt_image *img, float **dev);
+/* Image(const RenderContext *) (compositor.cpp:12-20): the scene's last path-traced frame. */
+int rt_image_from_scene(rt_image *img, rt_scene *scene);
+/* Image::Convolute (compositor.h:107-146): rows x cols weights, row-major, taps centred at rows/2, cols/2. */
+int rt_image_convolve(rt_image *img, uint32_t rows, uint32_t cols, const float *weights, uint32_t flags);
+/* Image::GaussianBlur (compositor.cpp:42-49): UnitizedGaussianKernel X pass, then Y pass. */
+int rt_image_gaussian_blur(rt_image *img, uint32_t size, float sigma);
+/* Image::Bloom (compositor.cpp:70-78): BT.709 luma above threshold, blurred, added back. */
+int rt_image_bloom(rt_image *img, uint32_t size, float sigma, float threshold);
+/* Image::Tonemap<standard> (compositor.h:221-228); standard as Convert::ColorStandard (0 = BT.709). */
+int rt_image_tonemap(rt_image *img, float limit, int standard);
+/* Image::GammaCorrection (compositor.cpp:95-101). */
+int rt_image_gamma(rt_image *img, float gamma);
+/* Image::GreyScale<standard>, Image::Binarize<standard> (compositor.h:184-216). */
+int rt_image_greyscale(rt_image *img, int standard);
+int rt_image_binarize(rt_image *img, int standard, float threshold);
+/* Image::EdgeDetection<orientation> (compositor.h:154-180), Sobel; orientation as
+ * RenderToy::Orientation: 0 X, 1 Y, 2 All. */
+int rt_image_edge_detect(rt_image *img, int orientation);
+/* dst = a op b (Image::operator+ - * /, compositor.cpp:103-161); op 0..3; dst may be a or b. */
+int rt_image_arith(rt_image *dst, const rt_image *a, const rt_image *b, int op);
 
 /* Device pointer of a pass left on the device by the last render, for callers
  * that keep results in HBM (pass = one RT_AOV_* bit, or 0 for the path-traced

@@ -36,7 +36,38 @@ def _u(a):
     return np.ascontiguousarray(a, np.uint32)
 
 
-class Oracle:
+# Image operations shared by the oracle and the reference, by name:
+# (reference op code of ref_image_apply, size used, a, b meaning)
+IMAGE_OPS = {"blur": 0, "bloom": 1, "tonemap": 2, "gamma": 3, "grey709": 4, "binarize709": 5,
+             "edge_x": 6, "edge_y": 7, "edge_all": 8, "grey601": 9, "tonemap_avg": 10}
+
+
+def _img(a):
+    return np.ascontiguousarray(a, np.float32).copy()
+
+
+class _ImageMixin:
+    """Image operations on float32 (H, W, 3) arrays; returns new arrays."""
+
+    def gaussian_weights(self, size, sigma, orient):
+        out = np.zeros(size * size if orient == 0 else size, np.float32)
+        self._gauss(size, sigma, orient, out.ctypes.data)
+        return out
+
+    def image_convolve(self, img, k, flags=0):
+        x = _img(img)
+        k = np.ascontiguousarray(k, np.float32)
+        self._conv(x.ctypes.data, x.shape[1], x.shape[0], k.shape[0], k.shape[1], k.ctypes.data, flags)
+        return x
+
+    def image_arith(self, a, b, op):
+        a, b = _img(a), _img(b)
+        out = np.zeros_like(a)
+        self._arith(a.ctypes.data, b.ctypes.data, out.ctypes.data, a.shape[1], a.shape[0], op)
+        return out
+
+
+class Oracle(_ImageMixin):
     def __init__(self, path=ORACLE_SO):
         if not os.path.exists(path):
             raise FileNotFoundError("oracle library not built: %s (run __graft_entry__.build())" % path)
@@ -57,6 +88,19 @@ class Oracle:
         L.orc_bsdf_sample.argtypes = [vp, C.c_float, vp, vp, C.c_float, C.c_float, vp, vp, vp]
         L.orc_bsdf_sample.restype = C.c_int
         L.orc_philox4x32_10.argtypes = [vp, vp, vp]
+        L.orc_gaussian_weights.argtypes = [C.c_uint32, C.c_float, C.c_int, vp]
+        L.orc_image_convolve.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int]
+        L.orc_image_gaussian_blur.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float]
+        L.orc_image_bloom.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+        L.orc_image_tonemap.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int]
+        L.orc_image_gamma.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_float]
+        L.orc_image_greyscale.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int]
+        L.orc_image_binarize.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, C.c_float]
+        L.orc_image_edge_detect.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int]
+        L.orc_image_arith.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_int]
+        self._gauss = L.orc_gaussian_weights
+        self._conv = L.orc_image_convolve
+        self._arith = L.orc_image_arith
         L.orc_render_pt.argtypes = [vp, C.c_int, vp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                     C.c_uint64, C.c_uint64, vp, C.c_int, vp]
@@ -102,6 +146,24 @@ class Oracle:
         out = np.zeros(4, np.uint32)
         self.lib.orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(out))
         return out
+
+
+    def image(self, op, img, size=0, a=0.0, b=0.0):
+        x = _img(img)
+        L, W, H, d = self.lib, x.shape[1], x.shape[0], x.ctypes.data
+        if op == "blur": L.orc_image_gaussian_blur(d, W, H, size, a)
+        elif op == "bloom": L.orc_image_bloom(d, W, H, size, a, b)
+        elif op == "tonemap": L.orc_image_tonemap(d, W, H, a, 0)
+        elif op == "tonemap_avg": L.orc_image_tonemap(d, W, H, a, 4)
+        elif op == "gamma": L.orc_image_gamma(d, W, H, a)
+        elif op == "grey709": L.orc_image_greyscale(d, W, H, 0)
+        elif op == "grey601": L.orc_image_greyscale(d, W, H, 1)
+        elif op == "binarize709": L.orc_image_binarize(d, W, H, 0, a)
+        elif op == "edge_x": L.orc_image_edge_detect(d, W, H, 1)
+        elif op == "edge_y": L.orc_image_edge_detect(d, W, H, 2)
+        elif op == "edge_all": L.orc_image_edge_detect(d, W, H, 0)
+        else: raise KeyError(op)
+        return x
 
 
 class OracleScene:
@@ -197,7 +259,7 @@ class OracleScene:
         return out
 
 
-class RefLib:
+class RefLib(_ImageMixin):
     """RenderToy itself (oracle/_ref/libref.so)."""
 
     def __init__(self, path=REF_SO):
@@ -220,6 +282,13 @@ class RefLib:
         L.ref_bsdf_sample.argtypes = [vp, vp, C.c_float, vp, vp, vp, vp, vp, vp]
         L.ref_translation_rotation.argtypes = [vp, vp, vp]
         L.ref_world_destroy.argtypes = [vp]
+        L.ref_gaussian_weights.argtypes = [C.c_uint32, C.c_float, C.c_int, vp]
+        L.ref_image_convolve.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int]
+        L.ref_image_apply.argtypes = [C.c_int, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+        L.ref_image_arith.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_int]
+        self._gauss = L.ref_gaussian_weights
+        self._conv = L.ref_image_convolve
+        self._arith = L.ref_image_arith
 
     def world(self, sc):
         return RefWorld(self, sc)
@@ -236,6 +305,11 @@ class RefLib:
         t, e = _f(t), _f(euler)
         self.lib.ref_translation_rotation(_ptr(t), _ptr(e), _ptr(out))
         return out
+
+    def image(self, op, img, size=0, a=0.0, b=0.0):
+        x = _img(img)
+        self.lib.ref_image_apply(IMAGE_OPS[op], x.ctypes.data, x.shape[1], x.shape[0], size, a, b)
+        return x
 
 
 class RefWorld:

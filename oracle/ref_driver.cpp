@@ -12,7 +12,9 @@
  */
 #include <chrono>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <new>
 #include <optional>
 #include <random>
 #include <unordered_map>
@@ -25,6 +27,27 @@
 #undef protected
 
 using namespace RenderToy;
+
+/* Array allocations made inside libref get a zero-filled guard band on both
+ * sides (the library is linked -Bsymbolic, so the reference's own new[]
+ * resolves here).  Image's reflect edge mode reads up to a row past an image
+ * buffer (compositor.h:26-40: coordinate n maps to n), which is undefined
+ * behaviour; with the guard band those reads return black, the value the
+ * oracle defines for them.  Nothing else in the reference reads outside its
+ * arrays, so its results are unchanged. */
+static constexpr std::size_t kGuard = 1u << 20;   // bytes on each side
+
+void *operator new[](std::size_t n) {
+    char *p = static_cast<char *>(std::malloc(n + 2 * kGuard));
+    if (!p) throw std::bad_alloc();
+    std::memset(p, 0, kGuard);
+    std::memset(p + kGuard + n, 0, kGuard);
+    return p + kGuard;
+}
+void operator delete[](void *p) noexcept {
+    if (p) std::free(static_cast<char *>(p) - kGuard);
+}
+void operator delete[](void *p, std::size_t) noexcept { operator delete[](p); }
 
 namespace {
 struct RefWorld {
@@ -228,6 +251,91 @@ void ref_world_destroy(void *wp) {
     w->rc = nullptr;
     /* World owns raw pointers and never frees them (world.h:13-41); leak as the reference does. */
     delete w;
+}
+
+
+/* ---- Image (include/RenderToy/compositor.h, src/compositor.cpp) ---------- */
+
+static void img_in(Image &img, const float *src) {
+    const std::size_t n = img.resolution.Area();
+    for (std::size_t i = 0; i < n; ++i) img.GetBuffer()[i] = Vector3f(src[3 * i], src[3 * i + 1], src[3 * i + 2]);
+}
+static void img_out(const Image &img, float *dst) {
+    const std::size_t n = img.resolution.Area();
+    for (std::size_t i = 0; i < n; ++i) {
+        const Vector3f &v = img.GetBuffer()[i];
+        dst[3 * i] = v.x();
+        dst[3 * i + 1] = v.y();
+        dst[3 * i + 2] = v.z();
+    }
+}
+
+/* UnitizedGaussianKernel<float, _O> weights; orient 0 All, 1 X, 2 Y. */
+void ref_gaussian_weights(uint32_t size, float sigma, int orient, float *out) {
+    if (orient == 0) {
+        UnitizedGaussianKernel<float, Orientation::All> k(size, sigma);
+        for (uint32_t i = 0; i < size; ++i)
+            for (uint32_t j = 0; j < size; ++j) out[i * size + j] = k.kernel_mat[i][j];
+    } else if (orient == 1) {
+        UnitizedGaussianKernel<float, Orientation::X> k(size, sigma);
+        for (uint32_t j = 0; j < size; ++j) out[j] = k.kernel_mat[0][j];
+    } else {
+        UnitizedGaussianKernel<float, Orientation::Y> k(size, sigma);
+        for (uint32_t i = 0; i < size; ++i) out[i] = k.kernel_mat[i][0];
+    }
+}
+
+/* Image::Convolute<_Normalize, _Length, _Abs> with a rows x cols kernel;
+ * flags: 1 normalise, 2 length, 4 abs. */
+void ref_image_convolve(float *data, uint32_t W, uint32_t H, uint32_t rows, uint32_t cols, const float *k,
+                        int flags) {
+    Image img(SizeN(W, H));
+    img_in(img, data);
+    IConvolutionKernel<float> ker(rows, cols);
+    for (uint32_t i = 0; i < rows; ++i)
+        for (uint32_t j = 0; j < cols; ++j) ker.kernel_mat[i][j] = k[i * cols + j];
+    switch (flags & 7) {
+    case 0: img.Convolute<false, false, false>(ker); break;
+    case 1: img.Convolute<true, false, false>(ker); break;
+    case 2: img.Convolute<false, true, false>(ker); break;
+    case 3: img.Convolute<true, true, false>(ker); break;
+    case 4: img.Convolute<false, false, true>(ker); break;
+    case 5: img.Convolute<true, false, true>(ker); break;
+    case 6: img.Convolute<false, true, true>(ker); break;
+    default: img.Convolute<true, true, true>(ker); break;
+    }
+    img_out(img, data);
+}
+
+/* op: 0 GaussianBlur(size, a), 1 Bloom(size, a, b), 2 Tonemap(a), 3
+ * GammaCorrection(a), 4 GreyScale<BT709>, 5 Binarize<BT709>(a), 6/7/8
+ * EdgeDetection<X/Y/All>, 9 GreyScale<BT601>, 10 Tonemap<kAverage>(a). */
+void ref_image_apply(int op, float *data, uint32_t W, uint32_t H, uint32_t size, float a, float b) {
+    Image img(SizeN(W, H));
+    img_in(img, data);
+    switch (op) {
+    case 0: img.GaussianBlur(size, a); break;
+    case 1: img.Bloom(size, a, b); break;
+    case 2: img.Tonemap(a); break;
+    case 3: img.GammaCorrection(a); break;
+    case 4: img.GreyScale<Convert::ColorStandard::kITURBT709>(); break;
+    case 5: img.Binarize<Convert::ColorStandard::kITURBT709>(a); break;
+    case 6: img.EdgeDetection<Orientation::X>(); break;
+    case 7: img.EdgeDetection<Orientation::Y>(); break;
+    case 8: img.EdgeDetection<Orientation::All>(); break;
+    case 9: img.GreyScale<Convert::ColorStandard::kITURBT601>(); break;
+    default: img.Tonemap<Convert::ColorStandard::kAverage>(a); break;
+    }
+    img_out(img, data);
+}
+
+/* Image::operator+ - * / ; op 0..3. */
+void ref_image_arith(const float *a, const float *b, float *out, uint32_t W, uint32_t H, int op) {
+    Image ia(SizeN(W, H)), ib(SizeN(W, H));
+    img_in(ia, a);
+    img_in(ib, b);
+    Image r = op == 0 ? ia + ib : op == 1 ? ia - ib : op == 2 ? ia * ib : ia / ib;
+    img_out(r, out);
 }
 
 }  // extern "C"

@@ -21,7 +21,7 @@ RT_OK = 0
 RT_ERR = {1: "RT_ERR_INVALID", 2: "RT_ERR_HIP", 3: "RT_ERR_NO_DEVICE", 4: "RT_ERR_OOM", 5: "RT_ERR_INTERNAL"}
 AOV_ID, AOV_T, AOV_NORMAL, AOV_DEPTH, AOV_ALBEDO, AOV_MASK = 1, 2, 4, 8, 16, 32
 AOV_ALL = 63
-PT_NO_GRAPH, PT_NO_SORT, PT_NO_TUNE = 1, 2, 4
+PT_NO_GRAPH, PT_NO_TUNE = 1, 4
 NONE = 0xFFFFFFFF
 
 
@@ -122,6 +122,21 @@ def lib():
     L.rt_batch_destroy.argtypes = [vp]
     L.rt_batch_render.argtypes = [vp, C.POINTER(rt_camera), C.POINTER(rt_pt_params), C.POINTER(vp)]
     L.rt_batch_stats.argtypes = [vp, C.POINTER(rt_stats)]
+    L.rt_image_create.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(vp)]
+    L.rt_image_destroy.argtypes = [vp]
+    L.rt_image_upload.argtypes = [vp, vp]
+    L.rt_image_download.argtypes = [vp, vp]
+    L.rt_image_device_buffer.argtypes = [vp, C.POINTER(vp)]
+    L.rt_image_from_scene.argtypes = [vp, vp]
+    L.rt_image_convolve.argtypes = [vp, C.c_uint32, C.c_uint32, vp, C.c_uint32]
+    L.rt_image_gaussian_blur.argtypes = [vp, C.c_uint32, C.c_float]
+    L.rt_image_bloom.argtypes = [vp, C.c_uint32, C.c_float, C.c_float]
+    L.rt_image_tonemap.argtypes = [vp, C.c_float, C.c_int]
+    L.rt_image_gamma.argtypes = [vp, C.c_float]
+    L.rt_image_greyscale.argtypes = [vp, C.c_int]
+    L.rt_image_binarize.argtypes = [vp, C.c_int, C.c_float]
+    L.rt_image_edge_detect.argtypes = [vp, C.c_int]
+    L.rt_image_arith.argtypes = [vp, vp, vp, C.c_int]
     _lib = L
     return L
 
@@ -371,3 +386,86 @@ class Batch:
         s = rt_stats()
         check(lib().rt_batch_stats(self.h, C.byref(s)))
         return s.as_dict()
+
+
+class Image:
+    """An rt_image: RenderToy's Image (compositor.h) in device memory, (H, W, 3) float32."""
+
+    def __init__(self, width, height):
+        h = C.c_void_p()
+        check(lib().rt_image_create(width, height, C.byref(h)))
+        self.h, self.width, self.height = h, width, height
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a, np.float32)
+        img = cls(a.shape[1], a.shape[0])
+        img.upload(a)
+        return img
+
+    @classmethod
+    def from_scene(cls, scene, width, height):
+        img = cls(width, height)
+        check(lib().rt_image_from_scene(img.h, scene.h))
+        return img
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rt_image_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        if a.size != self.width * self.height * 3:
+            raise ValueError("array does not match the image size")
+        check(lib().rt_image_upload(self.h, a.ctypes.data))
+        return self
+
+    def array(self):
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        check(lib().rt_image_download(self.h, out.ctypes.data))
+        return out
+
+    def convolve(self, k, flags=0):
+        k = np.ascontiguousarray(k, np.float32)
+        check(lib().rt_image_convolve(self.h, k.shape[0], k.shape[1], k.ctypes.data, flags))
+        return self
+
+    def gaussian_blur(self, size, sigma):
+        check(lib().rt_image_gaussian_blur(self.h, size, sigma))
+        return self
+
+    def bloom(self, size, sigma, threshold):
+        check(lib().rt_image_bloom(self.h, size, sigma, threshold))
+        return self
+
+    def tonemap(self, limit=1.5, standard=0):
+        check(lib().rt_image_tonemap(self.h, limit, standard))
+        return self
+
+    def gamma(self, gamma=2.2):
+        check(lib().rt_image_gamma(self.h, gamma))
+        return self
+
+    def greyscale(self, standard=0):
+        check(lib().rt_image_greyscale(self.h, standard))
+        return self
+
+    def binarize(self, threshold, standard=0):
+        check(lib().rt_image_binarize(self.h, standard, threshold))
+        return self
+
+    def edge_detect(self, orientation=2):
+        check(lib().rt_image_edge_detect(self.h, orientation))
+        return self
+
+    def arith(self, a, b, op):
+        """self = a op b (0 +, 1 -, 2 *, 3 /)."""
+        check(lib().rt_image_arith(self.h, a.h, b.h, op))
+        return self

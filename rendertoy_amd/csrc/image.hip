@@ -1,0 +1,462 @@
+// image.hip -- RenderToy's Image operations on the device (rt_image_*).
+//
+// Restates Image (include/RenderToy/compositor.h, src/compositor.cpp): an
+// image is float[height][width][3] in HBM, row-major like RenderContext::buffer.
+// Every output pixel is computed by one thread in the reference's operation
+// order (taps in row-major order, one IEEE operation at a time, built with
+// -ffp-contract=off), so results are bit-identical to the reference wherever
+// the reference reads inside its buffer.  Edge reads: BufferGetWrapper<kReflect>
+// (compositor.h:26-40) reflects a coordinate once and reads buffer[x*W + y];
+// coordinate n maps to n, which is the next row's first pixel, or past the
+// buffer's end on the last row or column (undefined behaviour in the
+// reference).  A flat index inside the buffer reads that pixel; one outside
+// reads black.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rendertoy_amd.h"
+#include "scene.h"
+
+struct rt_image {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t width = 0, height = 0;
+    float *d = nullptr;        // the image
+    float *tmp = nullptr;      // scratch image of the same size (convolution source, bloom layer)
+    float *k = nullptr;        // convolution weights
+    size_t k_cap = 0;          // floats the weight buffer holds
+    ~rt_image();
+};
+
+namespace rtamd {
+namespace {
+
+constexpr uint32_t kLdsTaps = 8192;   // weights kept in LDS up to this many taps
+
+__device__ __forceinline__ float3 ld3f(const float *p, int64_t i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void st3f(float *p, int64_t i, float3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
+__device__ __forceinline__ float3 reflect_read(const float *img, int W, int H, int px, int py) {
+    if (px < 0) px = -px;
+    if (py < 0) py = -py;
+    if (py >= W) py = 2 * W - py;
+    if (px >= H) px = 2 * H - px;
+    const int64_t flat = (int64_t)px * W + py;
+    if (flat < 0 || flat >= (int64_t)W * H) return make_float3(0.0f, 0.0f, 0.0f);
+    return ld3f(img, flat);
+}
+
+// Convert::Luma (rtmath.h:1140-1166); standard as Convert::ColorStandard.
+__device__ __forceinline__ float luma(float3 v, int standard) {
+    switch (standard) {
+    case 1: return 0.299f * v.x + 0.587f * v.y + 0.114f * v.z;
+    case 2: return 0.2627f * v.x + 0.6780f * v.y + 0.0593f * v.z;
+    case 3: return 0.2122f * v.x + 0.7013f * v.y + 0.0865f * v.z;
+    case 4: return (v.x + v.y + v.z) / 3.0f;
+    default: return 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z;
+    }
+}
+
+__device__ __forceinline__ float length3(float3 v) { return sqrtf(0.0f + v.x * v.x + v.y * v.y + v.z * v.z); }
+
+// Image::Convolute<_Normalize, _Length, _Abs> (compositor.h:107-146).  Block
+// 64 x 4 threads: a wave covers 64 consecutive columns of one row, so every
+// tap is one coalesced read.
+__global__ __launch_bounds__(256) void conv_kernel(const float *src, float *dst, int W, int H, int rows, int cols,
+                                                   const float *k, int flags) {
+    __shared__ float s_k[kLdsTaps];
+    const int ntaps = rows * cols;
+    const bool in_lds = ntaps <= (int)kLdsTaps;
+    if (in_lds)
+        for (int t = threadIdx.x; t < ntaps; t += blockDim.x) s_k[t] = k[t];
+    __syncthreads();
+    const float *kk = in_lds ? s_k : k;
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= H || j >= W) return;
+    const int r2 = rows >> 1, c2 = cols >> 1;
+    float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+    for (int ii = 0; ii < rows; ++ii)
+        for (int jj = 0; jj < cols; ++jj) {
+            const float3 v = reflect_read(src, W, H, i - r2 + ii, j - c2 + jj);
+            const float w = kk[ii * cols + jj];
+            acc.x = acc.x + v.x * w;
+            acc.y = acc.y + v.y * w;
+            acc.z = acc.z + v.z * w;
+        }
+    if (flags & RT_CONV_ABS) acc = make_float3(fabsf(acc.x), fabsf(acc.y), fabsf(acc.z));
+    if (flags & RT_CONV_LENGTH) {
+        const float l = length3(acc);
+        acc = make_float3(l, l, l);
+    }
+    if (flags & RT_CONV_NORMALIZE) {
+        const float l = length3(acc);
+        acc = make_float3(acc.x / l, acc.y / l, acc.z / l);
+    }
+    st3f(dst, (int64_t)i * W + j, acc);
+}
+
+// Per-pixel operations (compositor.h:184-228, compositor.cpp:70-101).
+enum PixOp { kExtract = 0, kTonemap, kGamma, kGrey, kBinarize, kEdgeCombine, kAdd, kSub, kMul, kDiv };
+
+__global__ void pixel_kernel(int op, const float *a, const float *b, float *out, int64_t n, float p0, int standard) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float3 v = ld3f(a, i);
+    float3 r;
+    switch (op) {
+    case kExtract: {   // bloom layer: pixels whose BT.709 luma exceeds p0
+        r = luma(v, 0) > p0 ? v : make_float3(0.0f, 0.0f, 0.0f);
+        break;
+    }
+    case kTonemap: {   // Convert::Tonemap: v * 1 / (1 + luma / limit)
+        const float s = 1.0f + luma(v, standard) / p0;
+        r = make_float3((v.x * 1.0f) / s, (v.y * 1.0f) / s, (v.z * 1.0f) / s);
+        break;
+    }
+    case kGamma:       // p0 = 1 / gamma
+        r = make_float3(powf(v.x, p0), powf(v.y, p0), powf(v.z, p0));
+        break;
+    case kGrey: {
+        const float l = luma(v, standard);
+        r = make_float3(l, l, l);
+        break;
+    }
+    case kBinarize: {
+        const float l = luma(v, standard) > p0 ? 1.0f : 0.0f;
+        r = make_float3(l, l, l);
+        break;
+    }
+    case kEdgeCombine: {   // EdgeDetection<All>: (Y result + X result).Length()
+        const float3 c = ld3f(b, i);
+        const float l = length3(make_float3(v.x + c.x, v.y + c.y, v.z + c.z));
+        r = make_float3(l, l, l);
+        break;
+    }
+    default: {
+        const float3 c = ld3f(b, i);
+        if (op == kAdd) r = make_float3(v.x + c.x, v.y + c.y, v.z + c.z);
+        else if (op == kSub) r = make_float3(v.x - c.x, v.y - c.y, v.z - c.z);
+        else if (op == kMul) r = make_float3(v.x * c.x, v.y * c.y, v.z * c.z);
+        else r = make_float3(v.x / c.x, v.y / c.y, v.z / c.z);
+        break;
+    }
+    }
+    st3f(out, i, r);
+}
+
+struct HipErr : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct BadArg : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+
+void chk(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw HipErr(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return RT_OK;
+    } catch (const BadArg &e) {
+        set_error(e.what());
+        return RT_ERR_INVALID;
+    } catch (const std::bad_alloc &) {
+        set_error("host out of memory");
+        return RT_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return RT_ERR_HIP;
+    }
+}
+
+int64_t npix(const rt_image *img) { return (int64_t)img->width * img->height; }
+
+void sync(rt_image *img) { chk(hipStreamSynchronize(img->stream), "hipStreamSynchronize"); }
+
+// src -> dst convolution with host weights (uploaded to the image's weight buffer)
+void convolve(rt_image *img, const float *src, float *dst, uint32_t rows, uint32_t cols, const float *w, int flags) {
+    const size_t n = (size_t)rows * cols;
+    if (n > img->k_cap) {
+        if (img->k) chk(hipFree(img->k), "hipFree");
+        img->k = nullptr;
+        chk(hipMalloc(&img->k, n * sizeof(float)), "hipMalloc");
+        img->k_cap = n;
+    }
+    chk(hipMemcpyAsync(img->k, w, n * sizeof(float), hipMemcpyHostToDevice, img->stream), "hipMemcpyAsync");
+    dim3 grid((img->width + 63) / 64, (img->height + 3) / 4);
+    hipLaunchKernelGGL(conv_kernel, grid, dim3(256), 0, img->stream, src, dst, (int)img->width, (int)img->height,
+                       (int)rows, (int)cols, img->k, flags);
+    chk(hipGetLastError(), "conv_kernel");
+    // the host weights must outlive the asynchronous copy
+    sync(img);
+}
+
+// Convolute in place: the reference convolves a copy (Image tmp(*this)) into the buffer.
+void convolve_inplace(rt_image *img, uint32_t rows, uint32_t cols, const float *w, int flags) {
+    chk(hipMemcpyAsync(img->tmp, img->d, (size_t)npix(img) * 12, hipMemcpyDeviceToDevice, img->stream),
+        "hipMemcpyAsync");
+    convolve(img, img->tmp, img->d, rows, cols, w, flags);
+}
+
+void pixel(rt_image *img, int op, const float *a, const float *b, float *out, float p0, int standard) {
+    const int64_t n = npix(img);
+    hipLaunchKernelGGL(pixel_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, img->stream, op, a, b, out, n,
+                       p0, standard);
+    chk(hipGetLastError(), "pixel_kernel");
+}
+
+// UnitizedGaussianKernel<float, X or Y> (rtmath.h:1019-1059), on the host with
+// the reference's float pow/exp so the weights are the reference's bits.
+std::vector<float> gaussian_1d(uint32_t size, float sigma) {
+    std::vector<float> w(size);
+    const float orig = float(size >> 1);
+    float sum = 0.0f;
+    for (uint32_t j = 0; j < size; ++j) {
+        const float d2 = std::pow(float(j) - orig, 2.0f);
+        const float g = std::exp(-(d2) / (2.0f * sigma * sigma));
+        sum += g;
+        w[j] = g;
+    }
+    for (uint32_t j = 0; j < size; ++j) w[j] /= sum;
+    return w;
+}
+
+// GaussianBlur (compositor.cpp:42-49) of the image buffer `buf` through `tmp`.
+void gaussian_blur(rt_image *img, float *buf, float *scratch, uint32_t size, float sigma) {
+    const std::vector<float> w = gaussian_1d(size, sigma);
+    const size_t bytes = (size_t)npix(img) * 12;
+    chk(hipMemcpyAsync(scratch, buf, bytes, hipMemcpyDeviceToDevice, img->stream), "hipMemcpyAsync");
+    convolve(img, scratch, buf, 1, size, w.data(), 0);
+    chk(hipMemcpyAsync(scratch, buf, bytes, hipMemcpyDeviceToDevice, img->stream), "hipMemcpyAsync");
+    convolve(img, scratch, buf, size, 1, w.data(), 0);
+}
+
+void check_img(const rt_image *img) {
+    if (!img) throw BadArg("image is NULL");
+}
+
+}  // namespace
+}  // namespace rtamd
+
+using namespace rtamd;
+
+rt_image::~rt_image() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (d) (void)hipFree(d);
+    if (tmp) (void)hipFree(tmp);
+    if (k) (void)hipFree(k);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+int rt_image_create(uint32_t width, uint32_t height, rt_image **out) {
+    if (!out) {
+        set_error("out is NULL");
+        return RT_ERR_INVALID;
+    }
+    *out = nullptr;
+    if (width == 0 || height == 0 || (uint64_t)width * height >= (1ull << 31)) {
+        set_error("bad image size");
+        return RT_ERR_INVALID;
+    }
+    rt_image *img = new (std::nothrow) rt_image();
+    if (!img) {
+        set_error("host out of memory");
+        return RT_ERR_OOM;
+    }
+    const int rc = guarded([&]() {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw BadArg("no HIP device visible");
+        chk(hipGetDevice(&img->device), "hipGetDevice");
+        img->width = width;
+        img->height = height;
+        chk(hipStreamCreateWithFlags(&img->stream, hipStreamNonBlocking), "hipStreamCreate");
+        const size_t bytes = (size_t)width * height * 12;
+        chk(hipMalloc(&img->d, bytes), "hipMalloc");
+        chk(hipMalloc(&img->tmp, bytes), "hipMalloc");
+        chk(hipMemsetAsync(img->d, 0, bytes, img->stream), "hipMemsetAsync");
+        sync(img);
+    });
+    if (rc != RT_OK) {
+        delete img;
+        return rc;
+    }
+    *out = img;
+    return RT_OK;
+}
+
+int rt_image_destroy(rt_image *img) {
+    delete img;
+    return RT_OK;
+}
+
+int rt_image_upload(rt_image *img, const float *rgb_host) {
+    return guarded([&]() {
+        check_img(img);
+        if (!rgb_host) throw BadArg("rgb_host is NULL");
+        chk(hipMemcpy(img->d, rgb_host, (size_t)npix(img) * 12, hipMemcpyHostToDevice), "hipMemcpy");
+    });
+}
+
+int rt_image_download(const rt_image *img, float *rgb_host) {
+    return guarded([&]() {
+        check_img(img);
+        if (!rgb_host) throw BadArg("rgb_host is NULL");
+        chk(hipStreamSynchronize(img->stream), "hipStreamSynchronize");
+        chk(hipMemcpy(rgb_host, img->d, (size_t)npix(img) * 12, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+int rt_image_device_buffer(rt_image *img, float **dev) {
+    return guarded([&]() {
+        check_img(img);
+        if (!dev) throw BadArg("dev is NULL");
+        *dev = img->d;
+    });
+}
+
+int rt_image_from_scene(rt_image *img, rt_scene *scene) {
+    return guarded([&]() {
+        check_img(img);
+        void *p = nullptr;
+        size_t bytes = 0;
+        if (rt_scene_device_output(scene, 0, &p, &bytes) != RT_OK) throw BadArg(rt_last_error());
+        if (bytes != (size_t)npix(img) * 12) throw BadArg("image size differs from the scene's last frame");
+        chk(hipMemcpyAsync(img->d, p, bytes, hipMemcpyDeviceToDevice, img->stream), "hipMemcpyAsync");
+        sync(img);
+    });
+}
+
+int rt_image_convolve(rt_image *img, uint32_t rows, uint32_t cols, const float *weights, uint32_t flags) {
+    return guarded([&]() {
+        check_img(img);
+        if (rows == 0 || cols == 0 || !weights) throw BadArg("kernel must be non-empty");
+        if (flags & ~7u) throw BadArg("unknown convolution flags");
+        convolve_inplace(img, rows, cols, weights, (int)flags);
+    });
+}
+
+int rt_image_gaussian_blur(rt_image *img, uint32_t size, float sigma) {
+    return guarded([&]() {
+        check_img(img);
+        if (size == 0) throw BadArg("kernel size must be >= 1");
+        gaussian_blur(img, img->d, img->tmp, size, sigma);
+    });
+}
+
+int rt_image_bloom(rt_image *img, uint32_t size, float sigma, float threshold) {
+    return guarded([&]() {
+        check_img(img);
+        if (size == 0) throw BadArg("kernel size must be >= 1");
+        float *layer = nullptr, *scratch = nullptr;
+        const size_t bytes = (size_t)npix(img) * 12;
+        chk(hipMalloc(&layer, bytes), "hipMalloc");
+        hipError_t e = hipMalloc(&scratch, bytes);
+        if (e != hipSuccess) {
+            (void)hipFree(layer);
+            chk(e, "hipMalloc");
+        }
+        try {
+            pixel(img, kExtract, img->d, nullptr, layer, threshold, 0);
+            gaussian_blur(img, layer, scratch, size, sigma);
+            pixel(img, kAdd, img->d, layer, img->d, 0.0f, 0);
+            sync(img);
+        } catch (...) {
+            (void)hipFree(layer);
+            (void)hipFree(scratch);
+            throw;
+        }
+        (void)hipFree(layer);
+        (void)hipFree(scratch);
+    });
+}
+
+int rt_image_tonemap(rt_image *img, float limit, int standard) {
+    return guarded([&]() {
+        check_img(img);
+        pixel(img, kTonemap, img->d, nullptr, img->d, limit, standard);
+        sync(img);
+    });
+}
+
+int rt_image_gamma(rt_image *img, float gamma) {
+    return guarded([&]() {
+        check_img(img);
+        pixel(img, kGamma, img->d, nullptr, img->d, 1.0f / gamma, 0);
+        sync(img);
+    });
+}
+
+int rt_image_greyscale(rt_image *img, int standard) {
+    return guarded([&]() {
+        check_img(img);
+        pixel(img, kGrey, img->d, nullptr, img->d, 0.0f, standard);
+        sync(img);
+    });
+}
+
+int rt_image_binarize(rt_image *img, int standard, float threshold) {
+    return guarded([&]() {
+        check_img(img);
+        pixel(img, kBinarize, img->d, nullptr, img->d, threshold, standard);
+        sync(img);
+    });
+}
+
+int rt_image_edge_detect(rt_image *img, int orientation) {
+    static const float kx[9] = {-1, 0, 1, -2, 0, 2, -1, 0, 1};   // SobelKernel<X> (rtmath.h:1083-1085)
+    static const float ky[9] = {1, 2, 1, 0, 0, 0, -1, -2, -1};   // SobelKernel<Y> (rtmath.h:1087-1089)
+    return guarded([&]() {
+        check_img(img);
+        if (orientation == 0 || orientation == 1) {
+            convolve_inplace(img, 3, 3, orientation == 0 ? kx : ky, 0);
+            return;
+        }
+        if (orientation != 2) throw BadArg("orientation must be 0 (X), 1 (Y) or 2 (All)");
+        // cpy = X result (into a fresh buffer), this = Y result, then (this + cpy).Length()
+        float *cx = nullptr;
+        const size_t bytes = (size_t)npix(img) * 12;
+        chk(hipMalloc(&cx, bytes), "hipMalloc");
+        try {
+            convolve(img, img->d, cx, 3, 3, kx, 0);
+            convolve_inplace(img, 3, 3, ky, 0);
+            pixel(img, kEdgeCombine, img->d, cx, img->d, 0.0f, 0);
+            sync(img);
+        } catch (...) {
+            (void)hipFree(cx);
+            throw;
+        }
+        (void)hipFree(cx);
+    });
+}
+
+int rt_image_arith(rt_image *dst, const rt_image *a, const rt_image *b, int op) {
+    return guarded([&]() {
+        check_img(dst);
+        check_img(a);
+        check_img(b);
+        if (a->width != b->width || a->height != b->height || dst->width != a->width || dst->height != a->height)
+            throw BadArg("images differ in size (ImageSizeNotMatchException)");
+        if (op < 0 || op > 3) throw BadArg("op must be 0 (+), 1 (-), 2 (*) or 3 (/)");
+        chk(hipStreamSynchronize(a->stream), "hipStreamSynchronize");
+        chk(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        pixel(dst, kAdd + op, a->d, b->d, dst->d, 0.0f, 0);
+        sync(dst);
+    });
+}
+
+}  // extern "C"

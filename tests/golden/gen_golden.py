@@ -13,7 +13,13 @@ reference draws from one process-global std::mt19937 (src/rtmath.cpp:355) and
 parallelises over samples with OpenMP, so only OMP_NUM_THREADS=1 gives a
 deterministic, race-free image.
 
-Usage: python tests/golden/gen_golden.py [part ...]   (parts: geometry aov pt bsdf camera converged)
+Image fixtures (part `image`) come from RenderToy's own Image
+(src/compositor.cpp) on seeded random images.  ref_driver.cpp gives the
+reference's array allocations a zero-filled guard band, so the reads its
+reflect edge mode makes past an image buffer (undefined behaviour) return
+black, the value the oracle and the device define for them.
+
+Usage: python tests/golden/gen_golden.py [part ...]   (parts: geometry aov pt bsdf camera converged image)
 """
 from __future__ import annotations
 
@@ -209,8 +215,51 @@ def _conv_child(spp, warmup, path):
     np.save(path, img)
 
 
+def image_inputs():
+    """Seeded test images: a (23 x 37, HDR), b (48 x 64, sparse bright spots), c (23 x 37, positive)."""
+    rng = np.random.default_rng(2024)
+    a = (rng.uniform(0, 1, (23, 37, 3)) ** 3 * 4).astype(np.float32)
+    a[5, 7] = [20.0, 18.0, 15.0]
+    b = (rng.uniform(0, 1, (48, 64, 3)) ** 6 * 1.5).astype(np.float32)
+    for y, x in rng.integers(0, (48, 64), (6, 2)):
+        b[y, x] = rng.uniform(5, 30, 3)
+    c = rng.uniform(0.5, 2.0, (23, 37, 3)).astype(np.float32)
+    k = rng.normal(size=(5, 7)).astype(np.float32)
+    return a, b, c, k
+
+
+# (name, op, input, size, a, b)
+IMAGE_CASES = [
+    ("blur9", "blur", "a", 9, 2.0, 0.0), ("blur21", "blur", "a", 21, 6.0, 0.0),
+    ("blur61_wide", "blur", "a", 61, 12.0, 0.0), ("bloom15", "bloom", "a", 15, 4.0, 0.8),
+    ("bloom31", "bloom", "b", 31, 8.0, 1.0), ("bloom120_wide", "bloom", "b", 120, 50.0, 1.0),
+    ("tonemap", "tonemap", "a", 0, 1.5, 0.0), ("tonemap_avg", "tonemap_avg", "a", 0, 1.2, 0.0),
+    ("gamma", "gamma", "a", 0, 2.2, 0.0), ("grey709", "grey709", "a", 0, 0.0, 0.0),
+    ("grey601", "grey601", "a", 0, 0.0, 0.0), ("binarize", "binarize709", "a", 0, 0.5, 0.0),
+    ("edge_x", "edge_x", "a", 0, 0.0, 0.0), ("edge_y", "edge_y", "a", 0, 0.0, 0.0),
+    ("edge_all", "edge_all", "a", 0, 0.0, 0.0),
+]
+
+
+def part_image():
+    R = ref()
+    a, b, c, k = image_inputs()
+    inp = {"a": a, "b": b, "c": c}
+    res = {"in_a": a, "in_b": b, "in_c": c, "kernel": k}
+    for name, op, src, size, pa, pb in IMAGE_CASES:
+        res[name] = R.image(op, inp[src], size=size, a=pa, b=pb)
+    for f in range(8):
+        res["conv_flags%d" % f] = R.image_convolve(a, k, f)
+    for op in range(4):
+        res["arith%d" % op] = R.image_arith(a, c, op)
+    for orient in range(3):
+        res["gauss9_%d" % orient] = R.gaussian_weights(9, 2.5, orient)
+    res["gauss500_x"] = R.gaussian_weights(500, 50.0, 1)
+    np.savez_compressed(out("image.npz"), **res)
+
+
 PARTS = {"geometry": part_geometry, "aov": part_aov, "pt": part_pt, "bsdf": part_bsdf, "camera": part_camera,
-         "converged": part_converged}
+         "converged": part_converged, "image": part_image}
 
 if __name__ == "__main__":
     a = sys.argv[1:]

@@ -1,0 +1,121 @@
+"""Device Image operations (rt_image_*, rendertoy_amd/csrc/image.hip).
+
+Against RenderToy's own results (tests/golden/image.npz): bit for bit, except
+gamma, whose powf may differ from glibc's by an ulp or two (tolerance: 2 ulp).
+At 1920x1080 with the demo's Bloom(500, 50, 1) the device is compared with the
+oracle (itself pinned to the reference in test_image_cpu.py), bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(os.path.join(HERE, "golden", "image.npz"))
+
+
+def bits_equal(a, b):
+    return np.array_equal(np.ascontiguousarray(a, np.float32).view(np.uint32),
+                          np.ascontiguousarray(b, np.float32).view(np.uint32))
+
+
+def ulp_diff(a, b):
+    ai = np.ascontiguousarray(a, np.float32).view(np.int32).astype(np.int64)
+    bi = np.ascontiguousarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(ai - bi)
+
+
+def run(op, img, size, a, b):
+    from rendertoy_amd import capi
+    im = capi.Image.from_array(img)
+    if op == "blur": im.gaussian_blur(size, a)
+    elif op == "bloom": im.bloom(size, a, b)
+    elif op == "tonemap": im.tonemap(a, 0)
+    elif op == "tonemap_avg": im.tonemap(a, 4)
+    elif op == "gamma": im.gamma(a)
+    elif op == "grey709": im.greyscale(0)
+    elif op == "grey601": im.greyscale(1)
+    elif op == "binarize709": im.binarize(a, 0)
+    elif op == "edge_x": im.edge_detect(0)
+    elif op == "edge_y": im.edge_detect(1)
+    elif op == "edge_all": im.edge_detect(2)
+    out = im.array()
+    im.close()
+    return out
+
+
+CASES = [("blur9", "blur", "a", 9, 2.0, 0.0), ("blur21", "blur", "a", 21, 6.0, 0.0),
+         ("blur61_wide", "blur", "a", 61, 12.0, 0.0), ("bloom15", "bloom", "a", 15, 4.0, 0.8),
+         ("bloom31", "bloom", "b", 31, 8.0, 1.0), ("bloom120_wide", "bloom", "b", 120, 50.0, 1.0),
+         ("tonemap", "tonemap", "a", 0, 1.5, 0.0), ("tonemap_avg", "tonemap_avg", "a", 0, 1.2, 0.0),
+         ("gamma", "gamma", "a", 0, 2.2, 0.0), ("grey709", "grey709", "a", 0, 0.0, 0.0),
+         ("grey601", "grey601", "a", 0, 0.0, 0.0), ("binarize", "binarize709", "a", 0, 0.5, 0.0),
+         ("edge_x", "edge_x", "a", 0, 0.0, 0.0), ("edge_y", "edge_y", "a", 0, 0.0, 0.0),
+         ("edge_all", "edge_all", "a", 0, 0.0, 0.0)]
+
+
+@pytest.mark.parametrize("name,op,src,size,a,b", CASES)
+def test_device_image_matches_reference(gold, name, op, src, size, a, b):
+    print("case", name, flush=True)
+    got = run(op, gold["in_" + src], size, a, b)
+    if op == "gamma":
+        assert ulp_diff(got, gold[name]).max() <= 2
+    else:
+        assert bits_equal(got, gold[name]), name
+
+
+def test_device_convolve_and_arith_match_reference(gold):
+    from rendertoy_amd import capi
+    for f in range(8):
+        im = capi.Image.from_array(gold["in_a"]).convolve(gold["kernel"], f)
+        assert bits_equal(im.array(), gold["conv_flags%d" % f]), f
+    a = capi.Image.from_array(gold["in_a"])
+    c = capi.Image.from_array(gold["in_c"])
+    for op in range(4):
+        d = capi.Image(a.width, a.height).arith(a, c, op)
+        assert bits_equal(d.array(), gold["arith%d" % op]), op
+    # in place: a = a + c
+    a.arith(a, c, 0)
+    assert bits_equal(a.array(), gold["arith0"])
+
+
+def test_device_demo_bloom_full_hd():
+    """CornellBox demo post-processing (main.cpp:63-66) at 1920x1080."""
+    from py_oracle import Oracle
+    from rendertoy_amd import capi
+    rng = np.random.default_rng(7)
+    img = (rng.uniform(0, 1, (1080, 1920, 3)) ** 8 * 3).astype(np.float32)
+    o = Oracle()
+    ref = o.image("gamma", o.image("tonemap", o.image("bloom", img, size=500, a=50.0, b=1.0), a=1.5), a=2.2)
+    bloom_only = o.image("bloom", img, size=500, a=50.0, b=1.0)
+    im = capi.Image.from_array(img).bloom(500, 50.0, 1.0)
+    assert bits_equal(im.array(), bloom_only)
+    im.tonemap(1.5).gamma(2.2)
+    assert ulp_diff(im.array(), ref).max() <= 2
+
+
+def test_image_from_scene_frame():
+    from helpers import scene
+    from rendertoy_amd import capi
+    s = capi.Scene(scene("cornell"))
+    rgb = s.render_pt(spp=2, max_depth=3, seed=4)
+    im = capi.Image.from_scene(s, 64, 64)
+    assert bits_equal(im.array().reshape(-1, 3), rgb)
+    with pytest.raises(capi.RtError):
+        capi.Image.from_scene(s, 32, 32)
+
+
+def test_image_errors():
+    from rendertoy_amd import capi
+    with pytest.raises(capi.RtError):
+        capi.Image(0, 4)
+    a, b = capi.Image(4, 4), capi.Image(5, 4)
+    with pytest.raises(capi.RtError):
+        capi.Image(4, 4).arith(a, b, 0)
+    with pytest.raises(capi.RtError):
+        a.edge_detect(3)

@@ -132,6 +132,67 @@ def bench_c3(n_frames=2):
             "one_scene_at_a_time_msamples_per_s": round(samples / seq / 1e6, 2)}
 
 
+def bench_post(with_cpu):
+    """CornellBox demo post-processing (demo/CornellBox/main.cpp:63-66) on a
+    1920x1080 HDR frame: Bloom(500, 50, 1), Tonemap(), GammaCorrection(),
+    on the device (rt_image) and with RenderToy's own Image on the host."""
+    from rendertoy_amd import capi
+    rng = np.random.default_rng(7)
+    img = (rng.uniform(0, 1, (1080, 1920, 3)) ** 8 * 3).astype(np.float32)
+    im = capi.Image.from_array(img)
+
+    def gpu():
+        im.upload(img)
+        t0 = time.perf_counter()
+        im.bloom(500, 50.0, 1.0).tonemap(1.5).gamma(2.2)
+        return time.perf_counter() - t0
+    gpu()
+    g = min(gpu() for _ in range(3))
+    out = {"workload": "Bloom(500, 50, 1) + Tonemap + GammaCorrection, 1920x1080 (RenderToy CornellBox demo)",
+           "gpu_ms": round(g * 1e3, 3)}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import py_oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        try:
+            ref = py_oracle.RefLib()
+            t0 = time.perf_counter()
+            x = ref.image("bloom", img, size=500, a=50.0, b=1.0)
+            x = ref.image("tonemap", x, a=1.5)
+            ref.image("gamma", x, a=2.2)
+            c = time.perf_counter() - t0
+            out["cpu_baseline"] = {"ms": round(c * 1e3, 1), "cores": threads, "kind": "reference",
+                                   "sample": "the same full frame, RenderToy's Image (OpenMP convolution)"}
+            out["speedup_vs_cpu"] = round(c / g, 1)
+        except (FileNotFoundError, OSError):
+            pass
+    im.close()
+    return out
+
+
+def bench_ingest():
+    """The terrain scene (2.4M triangles): rt_scene_create (triangle caches +
+    BVH4 build + upload) against rt_scene_load of its binary scene cache."""
+    import tempfile
+    from rendertoy_amd import capi, scenes
+    sc = scenes.terrain_c2()
+    t0 = time.perf_counter()
+    s = capi.Scene(sc)
+    create = time.perf_counter() - t0
+    build_ms = s.stats()["build_ms"]
+    path = os.path.join(tempfile.mkdtemp(), "c2.rtscene")
+    s.save(path)
+    size = os.path.getsize(path)
+    t0 = time.perf_counter()
+    s2 = capi.Scene.load(path, sc)
+    load = time.perf_counter() - t0
+    del s, s2
+    os.remove(path)
+    return {"workload": "terrain scene, %d triangles" % sc.num_triangles, "create_ms": round(create * 1e3, 1),
+            "bvh_build_ms": round(build_ms, 1), "cache_load_ms": round(load * 1e3, 1),
+            "cache_bytes": size}
+
+
 def _rmse(img, ref, ok):
     d = img.astype(np.float64)[ok] - ref[ok]
     return float(np.sqrt(np.mean(d * d)))
@@ -211,6 +272,21 @@ def cache_study(args):
     img = c.render_pt(spp=1024, max_depth=depth, seed=8)
     ok = ref_ok & np.isfinite(img).all(1)
     bias = _rmse(img, ref, ok)
+    # inference on a known query count: rt_cache_eval is the only caller of
+    # nrc_infer_kernel<0>, one launch per call (its duration comes from the kernel trace)
+    nq = 1 << 21
+    rng = np.random.default_rng(5)
+    v = sc.verts.reshape(-1, 3)
+    q = np.zeros((nq, 20), np.float32)
+    q[:, 0:3] = v.min(0) + (v.max(0) - v.min(0)) * rng.uniform(0, 1, (nq, 3))
+    dd = rng.normal(size=(nq, 3))
+    q[:, 3:6] = dd / np.linalg.norm(dd, axis=1, keepdims=True)
+    q[:, 6:9] = q[:, 3:6]
+    q[:, 9] = 0.5
+    q[:, 10:13] = 0.5
+    q[:, 13] = depth - 2
+    c.eval(q[:1024])
+    _, eval_ms = _timed(lambda: c.eval(q), reps=2)
     # full-resolution throughput, plain vs cached
     scf = scenes.terrain_c2()
     sf = capi.Scene(scf)
@@ -235,6 +311,7 @@ def cache_study(args):
                "cache": {"levels": 16, "table": 1 << 19, "mlp": "64-64x5-3 ReLU", "train_batch": 65536,
                          "train_calls": args.cache_train, "steps_per_call": 4}},
            "plain": plain, "cached": cached, "cached_1024spp_rmse": bias,
+           "eval_2m_queries_ms_with_pcie": round(eval_ms, 3),
            "train": {"seconds": round(train_s, 3), "call_ms_65536_paths_1_step": round(train_call_ms, 3),
                      "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None},
            "full_resolution_1920x1080": full}
@@ -362,6 +439,8 @@ def main():
         del s2
         pt["c3_batch"] = bench_c3()
         out["path_tracer"] = pt
+        out["post_process"] = bench_post(not args.no_cpu)
+        out["scene_ingest"] = bench_ingest()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline_c1(sc, args.cpu_seconds)
     else:

@@ -212,6 +212,36 @@ int rt_cache_layout(const rt_cache *cache, float *res, uint32_t *dense, float *b
 int rt_render_pt_cached(rt_scene *scene, rt_cache *cache, const rt_camera *cam, const rt_pt_params *params,
                         float *rgb_host);
 
+/* OBJ ingest straight into scene-description arrays: OBJModelImporter::Import
+ * (src/importer.cpp:29-146) without its per-triangle heap objects.  Triangles
+ * are in World::triangles order (faces in file order, fanned (v0, v_i, v_i+1)
+ * as Polygon::ConvertToTriangle does); meshes in the order their `o` lines
+ * appear.  The caller adds materials and emitters and passes the arrays to
+ * rt_scene_create.  A missing file is RT_ERR_INVALID (RenderToy imports
+ * nothing); faces before any `o` go to an unnamed mesh (RenderToy dereferences
+ * a null mesh).  Arrays stay valid until rt_obj_destroy.  Needs no device. */
+typedef struct rt_obj {
+    uint32_t num_triangles;
+    const float *vertices;          /* num_triangles x 9 */
+    const float *normals;           /* num_triangles x 9 (zero where the file gives none) */
+    const uint32_t *tri_mesh;       /* mesh of each triangle */
+    uint32_t num_meshes;
+    const uint8_t *mesh_smooth;     /* Mesh::smooth */
+    const char *const *mesh_names;  /* Mesh::name */
+    void *internal;
+} rt_obj;
+int rt_obj_load(const char *path, rt_obj **out);
+int rt_obj_destroy(rt_obj *obj);
+
+/* Binary scene cache (no counterpart in RenderToy, which rebuilds its octree
+ * for every RenderContext): the scene's device arrays -- BVH4 nodes, leaf
+ * triangles, shading data, materials, emitters -- as one checksummed file.
+ * rt_scene_load uploads them without recomputing triangle caches or building
+ * the BVH; renders of a loaded scene are bit-identical to the saved scene's.
+ * stats.build_ms of a loaded scene is the load time. */
+int rt_scene_save(const rt_scene *scene, const char *path);
+int rt_scene_load(const char *path, rt_scene **out);
+
 /* Counters and timings of the last render call on this scene. */
 typedef struct rt_stats {
     double total_ms;              /* device time of the whole call (HIP events) */
@@ -295,6 +325,20 @@ int rt_image_binarize(rt_image *img, int standard, float threshold);
 int rt_image_edge_detect(rt_image *img, int orientation);
 /* dst = a op b (Image::operator+ - * /, compositor.cpp:103-161); op 0..3; dst may be a or b. */
 int rt_image_arith(rt_image *dst, const rt_image *a, const rt_image *b, int op);
+
+/* The image as a file in memory (src/exporter.cpp), pixels quantised on the
+ * device as int(255.99 * clamp(c, 0, 1)):
+ *   RT_EXPORT_BMP   -- BMPExporter (exporter.cpp:34-78).  RenderToy writes each
+ *                      row's padding from a null pointer (exporter.cpp:93), which
+ *                      fails when the width is not a multiple of 4, and leaves
+ *                      the padding out of the header's file size; here the
+ *                      padding is zeros and is counted.  Identical bytes when
+ *                      the width is a multiple of 4.
+ *   RT_EXPORT_PPM   -- PPMExporter (exporter.cpp:8-27), P3 text.
+ *   RT_EXPORT_ASCII -- ASCIIExporter (exporter.cpp:81-95).
+ * *size receives the byte count; with out NULL nothing else is written. */
+enum rt_export_format { RT_EXPORT_BMP = 0, RT_EXPORT_PPM = 1, RT_EXPORT_ASCII = 2 };
+int rt_image_export(const rt_image *img, int format, uint8_t *out, size_t capacity, size_t *size);
 
 /* Device pointer of a pass left on the device by the last render, for callers
  * that keep results in HBM (pass = one RT_AOV_* bit, or 0 for the path-traced

@@ -286,6 +286,10 @@ class RefLib(_ImageMixin):
         L.ref_image_convolve.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, vp, C.c_int]
         L.ref_image_apply.argtypes = [C.c_int, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
         L.ref_image_arith.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_int]
+        L.ref_image_export.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, vp, C.c_uint64]
+        L.ref_image_export.restype = C.c_uint64
+        L.ref_obj_import.argtypes = [C.c_char_p, vp, vp, vp, C.c_uint64, vp, vp, C.c_uint64, C.c_uint64, vp]
+        L.ref_obj_import.restype = C.c_uint64
         self._gauss = L.ref_gaussian_weights
         self._conv = L.ref_image_convolve
         self._arith = L.ref_image_arith
@@ -310,6 +314,27 @@ class RefLib(_ImageMixin):
         x = _img(img)
         self.lib.ref_image_apply(IMAGE_OPS[op], x.ctypes.data, x.shape[1], x.shape[0], size, a, b)
         return x
+
+    def obj_import(self, path):
+        """RenderToy's OBJModelImporter::Import: (verts n x 9, norms n x 9, tri_mesh, smooth, names)."""
+        nm = C.c_uint64()
+        n = self.lib.ref_obj_import(path.encode(), None, None, None, 0, None, None, 0, 0, C.byref(nm))
+        v = np.zeros((n, 9), np.float32)
+        nn = np.zeros((n, 9), np.float32)
+        tm = np.zeros(n, np.uint32)
+        sm = np.zeros(max(nm.value, 1), np.uint8)
+        names = C.create_string_buffer(1 << 16)
+        self.lib.ref_obj_import(path.encode(), v.ctypes.data, nn.ctypes.data, tm.ctypes.data, n, sm.ctypes.data,
+                                names, nm.value, 1 << 16, C.byref(nm))
+        return v, nn, tm, sm[:nm.value], names.value.decode().split("\n")[:nm.value]
+
+    def image_export(self, img, fmt):
+        """RenderToy's exporter output as bytes; fmt 0 BMP, 1 PPM, 2 ASCII."""
+        x = _img(img)
+        n = self.lib.ref_image_export(x.ctypes.data, x.shape[1], x.shape[0], fmt, None, 0)
+        buf = (C.c_uint8 * n)()
+        self.lib.ref_image_export(x.ctypes.data, x.shape[1], x.shape[0], fmt, buf, n)
+        return bytes(buf)
 
 
 class RefWorld:

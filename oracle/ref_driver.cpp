@@ -17,6 +17,7 @@
 #include <new>
 #include <optional>
 #include <random>
+#include <sstream>
 #include <unordered_map>
 #include <vector>
 
@@ -336,6 +337,58 @@ void ref_image_arith(const float *a, const float *b, float *out, uint32_t W, uin
     img_in(ib, b);
     Image r = op == 0 ? ia + ib : op == 1 ? ia - ib : op == 2 ? ia * ib : ia / ib;
     img_out(r, out);
+}
+
+
+/* PPMExporter / BMPExporter / ASCIIExporter (src/exporter.cpp); format 0 BMP,
+ * 1 PPM, 2 ASCII.  Returns the byte count (written to out when it fits). */
+uint64_t ref_image_export(const float *data, uint32_t W, uint32_t H, int format, uint8_t *out, uint64_t cap) {
+    Image img(SizeN(W, H));
+    img_in(img, data);
+    std::ostringstream os;
+    if (format == 0) BMPExporter(img).Export(os);
+    else if (format == 1) PPMExporter(img).Export(os);
+    else ASCIIExporter(img).Export(os);
+    const std::string s = os.str();
+    if (out && s.size() <= cap) std::memcpy(out, s.data(), s.size());
+    return s.size();
+}
+
+
+/* OBJModelImporter::Import (src/importer.cpp:29-146) into a fresh World:
+ * world.triangles' object-space vertices and normals (9 floats each), the index
+ * of each triangle's mesh in world.meshes (or 0xFFFFFFFF), each mesh's smooth
+ * flag, and its name (names joined with '\n').  Returns the triangle count;
+ * arrays are written when the capacities suffice. */
+uint64_t ref_obj_import(const char *path, float *verts, float *norms, uint32_t *tri_mesh, uint64_t cap_tris,
+                        uint8_t *smooth, char *names, uint64_t cap_meshes, uint64_t cap_names, uint64_t *n_meshes) {
+    World w;
+    OBJModelImporter::Import(w, path);
+    const uint64_t n = w.triangles.size();
+    *n_meshes = w.meshes.size();
+    std::unordered_map<const Mesh *, uint32_t> mid;
+    for (size_t m = 0; m < w.meshes.size(); ++m) mid[w.meshes[m]] = (uint32_t)m;
+    if (n <= cap_tris)
+        for (uint64_t i = 0; i < n; ++i) {
+            const Triangle *t = w.triangles[i];
+            for (int k = 0; k < 3; ++k)
+                for (int c = 0; c < 3; ++c) {
+                    verts[9 * i + 3 * k + c] = t->vert[k][c];
+                    norms[9 * i + 3 * k + c] = t->norm[k][c];
+                }
+            auto it = mid.find(t->parent);
+            tri_mesh[i] = it == mid.end() ? 0xFFFFFFFFu : it->second;
+        }
+    if (w.meshes.size() <= cap_meshes) {
+        std::string all;
+        for (size_t m = 0; m < w.meshes.size(); ++m) {
+            smooth[m] = w.meshes[m]->smooth ? 1 : 0;
+            all += w.meshes[m]->name;
+            all += '\n';
+        }
+        if (all.size() < cap_names) std::memcpy(names, all.c_str(), all.size() + 1);
+    }
+    return n;
 }
 
 }  // extern "C"

@@ -69,6 +69,12 @@ class rt_stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class rt_obj(C.Structure):
+    _fields_ = [("num_triangles", C.c_uint32), ("vertices", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("tri_mesh", C.POINTER(C.c_uint32)), ("num_meshes", C.c_uint32),
+                ("mesh_smooth", C.POINTER(C.c_uint8)), ("mesh_names", C.POINTER(C.c_char_p)), ("internal", C.c_void_p)]
+
+
 class rt_cache_params(C.Structure):
     _fields_ = [("log2_table", C.c_uint32), ("base_resolution", C.c_uint32), ("max_resolution", C.c_uint32),
                 ("train_batch", C.c_uint32), ("train_paths", C.c_uint32), ("first_vertex", C.c_uint32),
@@ -137,6 +143,11 @@ def lib():
     L.rt_image_binarize.argtypes = [vp, C.c_int, C.c_float]
     L.rt_image_edge_detect.argtypes = [vp, C.c_int]
     L.rt_image_arith.argtypes = [vp, vp, vp, C.c_int]
+    L.rt_image_export.argtypes = [vp, C.c_int, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.rt_obj_load.argtypes = [C.c_char_p, C.POINTER(C.POINTER(rt_obj))]
+    L.rt_obj_destroy.argtypes = [C.POINTER(rt_obj)]
+    L.rt_scene_save.argtypes = [vp, C.c_char_p]
+    L.rt_scene_load.argtypes = [C.c_char_p, C.POINTER(vp)]
     _lib = L
     return L
 
@@ -151,6 +162,22 @@ def device_info():
     cu = C.c_int()
     check(lib().rt_device_info(buf, 64, C.byref(cu)))
     return buf.value.decode(), cu.value
+
+
+def load_obj(path):
+    """rt_obj_load: dict of vertices (n, 9), normals (n, 9), tri_mesh, mesh_smooth, mesh_names."""
+    p = C.POINTER(rt_obj)()
+    check(lib().rt_obj_load(os.fsencode(path), C.byref(p)))
+    try:
+        o = p.contents
+        n, m = o.num_triangles, o.num_meshes
+        arr = lambda ptr, cnt, dt: np.ctypeslib.as_array(ptr, (cnt,)).astype(dt) if cnt else np.zeros(0, dt)
+        return {"vertices": arr(o.vertices, 9 * n, np.float32).reshape(n, 9),
+                "normals": arr(o.normals, 9 * n, np.float32).reshape(n, 9),
+                "tri_mesh": arr(o.tri_mesh, n, np.uint32), "mesh_smooth": arr(o.mesh_smooth, m, np.uint8),
+                "mesh_names": [o.mesh_names[i].decode() for i in range(m)]}
+    finally:
+        lib().rt_obj_destroy(p)
 
 
 def make_camera(cam, W, H) -> rt_camera:
@@ -209,6 +236,21 @@ class Scene:
 
     def camera(self, W=None, H=None):
         return make_camera(self.sc.camera, W or self.sc.width, H or self.sc.height)
+
+    def save(self, path):
+        """rt_scene_save: the binary scene cache."""
+        check(lib().rt_scene_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path, sc=None):
+        """rt_scene_load; sc (a scenes.Scene) supplies the default camera and frame size."""
+        obj = cls.__new__(cls)
+        obj.sc = sc
+        obj._keep = None
+        h = C.c_void_p()
+        check(lib().rt_scene_load(os.fsencode(path), C.byref(h)))
+        obj.h = h
+        return obj
 
     def set_counting(self, on=True):
         check(lib().rt_scene_set_counting(self.h, 1 if on else 0))
@@ -469,3 +511,12 @@ class Image:
         """self = a op b (0 +, 1 -, 2 *, 3 /)."""
         check(lib().rt_image_arith(self.h, a.h, b.h, op))
         return self
+
+    def export(self, fmt):
+        """File bytes: fmt 'bmp', 'ppm' or 'ascii'."""
+        f = {"bmp": 0, "ppm": 1, "ascii": 2}[fmt]
+        n = C.c_size_t()
+        check(lib().rt_image_export(self.h, f, None, 0, C.byref(n)))
+        buf = (C.c_uint8 * n.value)()
+        check(lib().rt_image_export(self.h, f, buf, n.value, C.byref(n)))
+        return bytes(buf)

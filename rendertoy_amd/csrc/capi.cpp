@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -297,6 +299,7 @@ int rt_scene_create(const rt_scene_desc *desc, rt_scene **out) {
         }
         ds.share_rays = ds.share_prim = kSchedAuto;
         s->num_tris = n;
+        s->num_materials = desc->num_materials;
         s->bvh_nodes = (uint32_t)bvh.nodes.size();
         s->bvh_depth = bvh.depth;
         s->max_stack = bvh.max_stack;
@@ -304,6 +307,174 @@ int rt_scene_create(const rt_scene_desc *desc, rt_scene **out) {
         RT_HIP(hipMalloc(&s->d_counters, 8 * 32 * sizeof(uint32_t)));
         RT_HIP(hipMalloc(&s->d_stat, 8 * sizeof(unsigned long long)));
         s->stats.build_ms = bvh.ms;
+        s->stats.bvh_nodes = s->bvh_nodes;
+        s->stats.bvh_depth = s->bvh_depth;
+        s->stats.bvh_max_stack = s->max_stack;
+        return (int)RT_OK;
+    });
+    if (rc != RT_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return RT_OK;
+}
+
+// ---------------------------------------------------------------- binary scene cache
+//
+// File (little-endian): "RTAMDSCN", u32 version 1, then u32 num_tris,
+// num_nodes, num_materials, num_emitters, has_vnorm, bvh_depth, max_stack,
+// f32 sky[3], ground[3], bbox[6]; then the device arrays as uploaded: BVH4
+// nodes (128 B each), leaf triangles (3 float4 per triangle), geo (float4),
+// info (u32), vertex normals (9 floats per triangle, if any), materials
+// (rt_material), emitters (EmitterRec); then an FNV-1a 64 checksum of all
+// preceding bytes.  Loading uploads them as they are: no triangle caches, no BVH build.
+
+namespace {
+constexpr char kSceneMagic[8] = {'R', 'T', 'A', 'M', 'D', 'S', 'C', 'N'};
+constexpr uint32_t kSceneVersion = 1;
+
+struct SceneHeader {
+    char magic[8];
+    uint32_t version, num_tris, num_nodes, num_materials, num_emitters, has_vnorm, bvh_depth, max_stack;
+    float sky[3], ground[3], bbox[6];
+};
+
+uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+    const uint8_t *b = static_cast<const uint8_t *>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+struct SceneArrays {
+    size_t nodes, tris, geo, info, vnorm, mats, emit;   // bytes
+};
+
+SceneArrays array_sizes(const SceneHeader &h) {
+    const size_t n1 = h.num_tris ? h.num_tris : 1;
+    return {(size_t)h.num_nodes * sizeof(Node4), 3 * n1 * sizeof(float4), n1 * sizeof(float4), n1 * sizeof(uint32_t),
+            h.has_vnorm ? (size_t)h.num_tris * 9 * sizeof(float) : 0, (size_t)h.num_materials * sizeof(rt_material),
+            (h.num_emitters ? h.num_emitters : 1) * sizeof(EmitterRec)};
+}
+}  // namespace
+
+int rt_scene_save(const rt_scene *s, const char *path) {
+    return guarded([&]() {
+        if (!s || !path) throw InvalidArg{"scene and path are required"};
+        SceneHeader h;
+        std::memset(&h, 0, sizeof(h));
+        std::memcpy(h.magic, kSceneMagic, 8);
+        h.version = kSceneVersion;
+        h.num_tris = s->num_tris;
+        h.num_nodes = s->bvh_nodes;
+        h.num_materials = s->num_materials;
+        h.num_emitters = s->ds.num_emit;
+        h.has_vnorm = s->ds.vnorm ? 1u : 0u;
+        h.bvh_depth = s->bvh_depth;
+        h.max_stack = s->max_stack;
+        for (int k = 0; k < 3; ++k) {
+            h.sky[k] = s->ds.sky[k];
+            h.ground[k] = s->ds.ground[k];
+        }
+        for (int k = 0; k < 6; ++k) h.bbox[k] = s->bbox[k];
+        const SceneArrays a = array_sizes(h);
+        const void *src[7] = {s->ds.nodes, s->ds.tris, s->ds.geo, s->ds.info, s->ds.vnorm, s->ds.mats, s->ds.emit};
+        const size_t len[7] = {a.nodes, a.tris, a.geo, a.info, a.vnorm, a.mats, a.emit};
+        FILE *f = std::fopen(path, "wb");
+        if (!f) throw InvalidArg{std::string("cannot open ") + path + " for writing"};
+        uint64_t sum = fnv1a(1469598103934665603ull, &h, sizeof(h));
+        bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+        std::vector<uint8_t> buf;
+        for (int i = 0; i < 7 && ok; ++i) {
+            if (!len[i]) continue;
+            buf.resize(len[i]);
+            hipError_t e = hipMemcpy(buf.data(), src[i], len[i], hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                std::fclose(f);
+                throw HipError{e, "hipMemcpy"};
+            }
+            sum = fnv1a(sum, buf.data(), len[i]);
+            ok = std::fwrite(buf.data(), 1, len[i], f) == len[i];
+        }
+        ok = ok && std::fwrite(&sum, sizeof(sum), 1, f) == 1;
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) throw InvalidArg{std::string("write to ") + path + " failed"};
+        return (int)RT_OK;
+    });
+}
+
+int rt_scene_load(const char *path, rt_scene **out) {
+    if (!out || !path) {
+        set_error("path and out are required");
+        return RT_ERR_INVALID;
+    }
+    *out = nullptr;
+    rt_scene *s = new (std::nothrow) rt_scene();
+    if (!s) {
+        set_error("host out of memory");
+        return RT_ERR_OOM;
+    }
+    int rc = guarded([&]() {
+        const auto t0 = std::chrono::steady_clock::now();
+        FILE *f = std::fopen(path, "rb");
+        if (!f) throw InvalidArg{std::string("cannot open ") + path};
+        std::vector<uint8_t> data;
+        std::fseek(f, 0, SEEK_END);
+        const long total = std::ftell(f);
+        std::fseek(f, 0, SEEK_SET);
+        if (total > 0) {
+            data.resize((size_t)total);
+            if (std::fread(data.data(), 1, data.size(), f) != data.size()) data.clear();
+        }
+        std::fclose(f);
+        if (data.size() < sizeof(SceneHeader) + 8) throw InvalidArg{"not a scene file (too short)"};
+        SceneHeader h;
+        std::memcpy(&h, data.data(), sizeof(h));
+        if (std::memcmp(h.magic, kSceneMagic, 8) != 0) throw InvalidArg{"not a scene file (bad magic)"};
+        if (h.version != kSceneVersion) throw InvalidArg{"unsupported scene file version"};
+        const SceneArrays a = array_sizes(h);
+        const size_t payload = sizeof(h) + a.nodes + a.tris + a.geo + a.info + a.vnorm + a.mats + a.emit;
+        if (data.size() != payload + 8) throw InvalidArg{"scene file size does not match its header"};
+        uint64_t sum;
+        std::memcpy(&sum, data.data() + payload, 8);
+        if (fnv1a(1469598103934665603ull, data.data(), payload) != sum) throw InvalidArg{"scene file checksum mismatch"};
+        check_device(s);
+        RT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        for (auto &e : s->ev) RT_HIP(hipEventCreate(&e));
+        const uint8_t *p = data.data() + sizeof(h);
+        auto next = [&](size_t bytes) {
+            void *d = dev_upload(s, bytes ? p : nullptr, bytes);
+            p += bytes;
+            return d;
+        };
+        DevScene &ds = s->ds;
+        ds.nodes = (const Node4 *)next(a.nodes);
+        ds.tris = (const float4 *)next(a.tris);
+        ds.geo = (const float4 *)next(a.geo);
+        ds.info = (const uint32_t *)next(a.info);
+        ds.vnorm = a.vnorm ? (const float *)next(a.vnorm) : nullptr;
+        ds.mats = (const rt_material *)next(a.mats);
+        ds.emit = (const EmitterRec *)next(a.emit);
+        ds.num_emit = h.num_emitters;
+        ds.num_tris = h.num_tris;
+        for (int k = 0; k < 3; ++k) {
+            ds.sky[k] = h.sky[k];
+            ds.ground[k] = h.ground[k];
+        }
+        ds.share_rays = ds.share_prim = kSchedAuto;
+        for (int k = 0; k < 6; ++k) s->bbox[k] = h.bbox[k];
+        s->num_tris = h.num_tris;
+        s->num_materials = h.num_materials;
+        s->bvh_nodes = h.num_nodes;
+        s->bvh_depth = h.bvh_depth;
+        s->max_stack = h.max_stack;
+        s->build_ms = 0.0;
+        RT_HIP(hipMalloc(&s->d_counters, 8 * 32 * sizeof(uint32_t)));
+        RT_HIP(hipMalloc(&s->d_stat, 8 * sizeof(unsigned long long)));
+        s->stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         s->stats.bvh_nodes = s->bvh_nodes;
         s->stats.bvh_depth = s->bvh_depth;
         s->stats.bvh_max_stack = s->max_stack;

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -152,6 +153,39 @@ __global__ void pixel_kernel(int op, const float *a, const float *b, float *out,
     }
     }
     st3f(out, i, r);
+}
+
+// Exporters (src/exporter.cpp): int(255.99 * clamp(c, 0, 1)) per channel.
+// format 0: BMP rows (B, G, R, zero padding to 4 bytes); 1: packed R, G, B;
+// 2: ASCII-art character index of the BT.2020 luma (exporter.cpp:81-95).
+__device__ __forceinline__ uint8_t quant(float c) {
+    const float x = fminf(fmaxf(c, 0.0f), 1.0f);   // std::clamp(c, 0, 1) for non-NaN c
+    return (uint8_t)(int)(255.99f * x);
+}
+
+__global__ void export_kernel(const float *img, uint8_t *out, int W, int H, int format, int rowbytes,
+                              uint32_t nchars) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)W * H) return;
+    const int i = (int)(p / W), j = (int)(p % W);
+    const float3 v = ld3f(img, p);
+    if (format == 0) {
+        uint8_t *row = out + (int64_t)i * rowbytes;
+        row[3 * j] = quant(v.z);
+        row[3 * j + 1] = quant(v.y);
+        row[3 * j + 2] = quant(v.x);
+        if (j == W - 1)
+            for (int b = 3 * W; b < rowbytes; ++b) row[b] = 0;
+    } else if (format == 1) {
+        out[3 * p] = quant(v.x);
+        out[3 * p + 1] = quant(v.y);
+        out[3 * p + 2] = quant(v.z);
+    } else {
+        const float l = luma(v, 2);
+        const float c = fminf(fmaxf(l, 0.0f), 1.0f);
+        size_t idx = (size_t)(c * float(nchars - 1));
+        out[p] = (uint8_t)(idx > nchars ? nchars : idx);
+    }
 }
 
 struct HipErr : std::runtime_error {
@@ -456,6 +490,88 @@ int rt_image_arith(rt_image *dst, const rt_image *a, const rt_image *b, int op) 
         chk(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         pixel(dst, kAdd + op, a->d, b->d, dst->d, 0.0f, 0);
         sync(dst);
+    });
+}
+
+// Exporters (src/exporter.cpp).  BMP (BMPExporter, exporter.cpp:34-78): 54-byte
+// header, rows bottom-up as stored, B G R, each row zero-padded to 4 bytes.
+// RenderToy writes that padding from a null pointer (exporter.cpp:93), which
+// fails for widths that are not a multiple of 4, and its header's file size
+// leaves the padding out; here the padding is written and counted (the two
+// agree byte for byte when the width is a multiple of 4).  PPM (PPMExporter,
+// exporter.cpp:8-27): P3 text.  ASCII (ASCIIExporter, exporter.cpp:81-95).
+int rt_image_export(const rt_image *img, int format, uint8_t *out, size_t capacity, size_t *size) {
+    static const char kAscii[] =
+        "`^\",:;Il!i~+_-?][}{1)(|\\/tfjrxnuvczXYUJCLQ0OZmwqpdbkhao*#MW&8%B@$";
+    return guarded([&]() {
+        check_img(img);
+        if (!size) throw BadArg("size is NULL");
+        if (format < 0 || format > 2) throw BadArg("format must be 0 (BMP), 1 (PPM) or 2 (ASCII)");
+        const int W = (int)img->width, H = (int)img->height;
+        const int rowbytes = (3 * W + 3) & ~3;
+        const uint32_t nchars = (uint32_t)(sizeof(kAscii) - 1);
+        const size_t raw = format == 0 ? (size_t)rowbytes * H : format == 1 ? (size_t)W * H * 3 : (size_t)W * H;
+        uint8_t *d = nullptr;
+        chk(hipMalloc(&d, raw ? raw : 1), "hipMalloc");
+        std::vector<uint8_t> q(raw);
+        try {
+            chk(hipStreamSynchronize(img->stream), "hipStreamSynchronize");
+            const int64_t n = npix(img);
+            hipLaunchKernelGGL(export_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, img->stream, img->d, d,
+                               W, H, format, rowbytes, nchars);
+            chk(hipGetLastError(), "export_kernel");
+            chk(hipMemcpyAsync(q.data(), d, raw, hipMemcpyDeviceToHost, img->stream), "hipMemcpyAsync");
+            chk(hipStreamSynchronize(img->stream), "hipStreamSynchronize");
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipFree(d);
+        std::string text;
+        std::vector<uint8_t> head;
+        if (format == 0) {
+            head.resize(54, 0);
+            auto put32 = [&](int off, uint32_t v) { std::memcpy(head.data() + off, &v, 4); };
+            auto put16 = [&](int off, uint16_t v) { std::memcpy(head.data() + off, &v, 2); };
+            head[0] = 'B';
+            head[1] = 'M';
+            put32(2, (uint32_t)(54 + raw));
+            put32(10, 54);
+            put32(14, 40);
+            put32(18, (uint32_t)W);
+            put32(22, (uint32_t)H);
+            put16(26, 1);
+            put16(28, 24);
+            put32(38, 3780);
+            put32(42, 3780);
+        } else if (format == 1) {
+            text = "P3\n" + std::to_string(W) + " " + std::to_string(H) + "\n255\n";
+            text.reserve(text.size() + (size_t)W * H * 12);
+            for (size_t p = 0; p < (size_t)W * H; ++p) {
+                text += std::to_string(q[3 * p]);
+                text += ' ';
+                text += std::to_string(q[3 * p + 1]);
+                text += ' ';
+                text += std::to_string(q[3 * p + 2]);
+                text += '\n';
+            }
+        } else {
+            text.reserve((size_t)(W + 1) * H);
+            for (int i = H - 1; i >= 0; --i) {   // top row first
+                for (int j = 0; j < W; ++j) text += kAscii[q[(size_t)i * W + j]];
+                text += '\n';
+            }
+        }
+        const size_t total = format == 0 ? head.size() + raw : text.size();
+        *size = total;
+        if (!out) return;
+        if (capacity < total) throw BadArg("output buffer too small");
+        if (format == 0) {
+            std::memcpy(out, head.data(), head.size());
+            std::memcpy(out + head.size(), q.data(), raw);
+        } else {
+            std::memcpy(out, text.data(), text.size());
+        }
     });
 }
 

@@ -30,6 +30,7 @@ struct rt_scene {
     std::vector<rtamd::DevBuf> owned;   // scene geometry
     rtamd::DevScene ds{};
     uint32_t num_tris = 0;
+    uint32_t num_materials = 0;
     uint32_t bvh_nodes = 0, bvh_depth = 0, max_stack = 0;
     double build_ms = 0.0;
     float bbox[6] = {0, 0, 0, 0, 0, 0};   // scene bounds (min xyz, max xyz)

@@ -19,7 +19,10 @@ reference's array allocations a zero-filled guard band, so the reads its
 reflect edge mode makes past an image buffer (undefined behaviour) return
 black, the value the oracle and the device define for them.
 
-Usage: python tests/golden/gen_golden.py [part ...]   (parts: geometry aov pt bsdf camera converged image)
+OBJ fixtures (part `obj`) are what RenderToy's OBJModelImporter builds from
+OBJ texts written here (a scene of our own and quirk cases).
+
+Usage: python tests/golden/gen_golden.py [part ...]   (parts: geometry aov pt bsdf camera converged image obj)
 """
 from __future__ import annotations
 
@@ -255,11 +258,93 @@ def part_image():
     for orient in range(3):
         res["gauss9_%d" % orient] = R.gaussian_weights(9, 2.5, orient)
     res["gauss500_x"] = R.gaussian_weights(500, 50.0, 1)
+    # exporters: e is 16 x 24 (width a multiple of 4, where RenderToy's BMP rows
+    # need no padding), with values below 0 and above 1 to exercise the clamp;
+    # a (width 37) only as PPM and ASCII (RenderToy's BMP exporter fails there)
+    e = np.random.default_rng(77).uniform(-0.2, 1.2, (16, 24, 3)).astype(np.float32)
+    res["in_e"] = e
+    for f, tag in ((0, "bmp"), (1, "ppm"), (2, "ascii")):
+        res["export_e_" + tag] = np.frombuffer(R.image_export(e, f), np.uint8)
+    for f, tag in ((1, "ppm"), (2, "ascii")):
+        res["export_a_" + tag] = np.frombuffer(R.image_export(a, f), np.uint8)
     np.savez_compressed(out("image.npz"), **res)
 
 
+# OBJ texts of our own that exercise the importer's parsing rules.
+OBJ_QUIRKS = """# comment line
+mtllib x.mtl
+o Quad
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+vt 0 0
+vt 1 1
+vn 0 0 1
+s 1
+usemtl M
+f 1/1/1 2/2/1 3/1/1 4/2/1
+o  Pent   extra words
+v 2 0 0
+v 3 0 0
+v 3.5 1e0 0
+v 2.5 2 -0.5
+v 1.5 1 0
+s 0
+f   5//1   6//1    7//1 8//1 9//1
+f 5 6 7
+o Tri
+vn 0 1 0
+f 1//2 2//2 3//2
+f 1/1/2 2/x/2 3/1/2x
+f 7 8
+g group
+v -1 -1 -1
+f 10 1 2
+"""
+OBJ_STOP = """o A
+v 0 0 0
+v 1 0 0
+v 0 1 0
+f 1 2 3
+s off
+o B
+f 1 2 3
+"""
+
+
+def obj_inputs():
+    import tempfile
+    texts = {"quirks": OBJ_QUIRKS, "quirks_crlf": OBJ_QUIRKS.replace("\n", "\r\n"), "stop": OBJ_STOP}
+    d = tempfile.mkdtemp()
+    sc = scenes.material_zoo(16, 16, 1)
+    path = os.path.join(d, "zoo.obj")
+    scenes.write_obj(sc, path)
+    texts["zoo"] = open(path).read()
+    return texts
+
+
+def part_obj():
+    import tempfile
+    R = ref()
+    res = {}
+    d = tempfile.mkdtemp()
+    for name, text in obj_inputs().items():
+        path = os.path.join(d, name + ".obj")
+        with open(path, "w", newline="") as f:
+            f.write(text)
+        v, n, tm, sm, names = R.obj_import(path)
+        res[name + "_text"] = np.frombuffer(text.encode(), np.uint8)
+        res[name + "_verts"] = v
+        res[name + "_norms"] = n
+        res[name + "_tri_mesh"] = tm
+        res[name + "_smooth"] = sm
+        res[name + "_names"] = np.array(names)
+    np.savez_compressed(out("obj.npz"), **res)
+
+
 PARTS = {"geometry": part_geometry, "aov": part_aov, "pt": part_pt, "bsdf": part_bsdf, "camera": part_camera,
-         "converged": part_converged, "image": part_image}
+         "converged": part_converged, "image": part_image, "obj": part_obj}
 
 if __name__ == "__main__":
     a = sys.argv[1:]

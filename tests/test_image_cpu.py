@@ -74,3 +74,17 @@ def test_blur_closed_forms(orc):
     out = orc.image("blur", imp, size=9, a=2.0)
     w = orc.gaussian_weights(9, 2.0, 1).astype(np.float64)
     np.testing.assert_allclose(out[16:25, 16:25, 0], np.outer(w, w), rtol=1e-6, atol=1e-9)
+
+
+def test_export_fixtures_are_self_consistent(gold):
+    """The reference's BMP and PPM of one image carry the same quantised pixels
+    (int(255.99 clamp(c, 0, 1))), as a numpy restatement computes them."""
+    e = gold["in_e"]
+    q = (255.99 * np.clip(e, 0.0, 1.0)).astype(np.int32).astype(np.uint8)
+    bmp = bytes(gold["export_e_bmp"])
+    px = np.frombuffer(bmp[54:], np.uint8).reshape(16, 24, 3)[:, :, ::-1]
+    assert np.array_equal(px, q)
+    lines = bytes(gold["export_e_ppm"]).split(b"\n")
+    assert lines[:3] == [b"P3", b"24 16", b"255"]
+    rgb = np.array([[int(v) for v in l.split()] for l in lines[3:3 + 24 * 16]], np.uint8).reshape(16, 24, 3)
+    assert np.array_equal(rgb, q)

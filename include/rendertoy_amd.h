@@ -340,6 +340,30 @@ int rt_image_arith(rt_image *dst, const rt_image *a, const rt_image *b, int op);
 enum rt_export_format { RT_EXPORT_BMP = 0, RT_EXPORT_PPM = 1, RT_EXPORT_ASCII = 2 };
 int rt_image_export(const rt_image *img, int format, uint8_t *out, size_t capacity, size_t *size);
 
+/* A pass of the scene's last render as an image: 0 = the path-traced frame (as
+ * rt_image_from_scene), RT_AOV_NORMAL or RT_AOV_ALBEDO of the last rt_render_aov
+ * (the denoiser's guides). */
+int rt_image_from_scene_pass(rt_image *img, rt_scene *scene, uint32_t pass);
+
+/* Denoiser interface (no counterpart in RenderToy's library; demo/WithOIDN/
+ * main.cpp:59-70 binds OpenImageDenoise's "RT" filter with the beauty image,
+ * optional albedo and normal auxiliaries, the output and an hdr flag, and
+ * rt_denoise takes the same roles).  An external denoiser can be fed
+ * rt_image_device_buffer()s instead.  The built-in filter is the
+ * edge-avoiding a-trous wavelet transform (Dammertz et al., HPG 2010): per
+ * iteration k a 5x5 B3-spline kernel dilated by 2^k, each tap weighted by
+ * exp(-|dc|^2/sc^2 - |dn|^2/sn^2 - |da|^2/sa^2), sc halving every iteration;
+ * with hdr, colour distances use c/(1+c).  albedo and normal may be NULL.
+ * params NULL: 3 iterations, sigmas 0.12 / 0.1 / 0.1, hdr (at 4 spp the colour
+ * term has to stay tight, or fireflies and visible emitters spread). */
+typedef struct rt_denoise_params {
+    uint32_t iterations;
+    float sigma_color, sigma_normal, sigma_albedo;
+    uint32_t hdr;
+} rt_denoise_params;
+int rt_denoise(const rt_image *color, const rt_image *albedo, const rt_image *normal, rt_image *output,
+               const rt_denoise_params *params);
+
 /* Device pointer of a pass left on the device by the last render, for callers
  * that keep results in HBM (pass = one RT_AOV_* bit, or 0 for the path-traced
  * RGB frame). */

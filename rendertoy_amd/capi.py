@@ -69,6 +69,11 @@ class rt_stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class rt_denoise_params(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("sigma_color", C.c_float), ("sigma_normal", C.c_float),
+                ("sigma_albedo", C.c_float), ("hdr", C.c_uint32)]
+
+
 class rt_obj(C.Structure):
     _fields_ = [("num_triangles", C.c_uint32), ("vertices", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
                 ("tri_mesh", C.POINTER(C.c_uint32)), ("num_meshes", C.c_uint32),
@@ -144,6 +149,8 @@ def lib():
     L.rt_image_edge_detect.argtypes = [vp, C.c_int]
     L.rt_image_arith.argtypes = [vp, vp, vp, C.c_int]
     L.rt_image_export.argtypes = [vp, C.c_int, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.rt_image_from_scene_pass.argtypes = [vp, vp, C.c_uint32]
+    L.rt_denoise.argtypes = [vp, vp, vp, vp, C.POINTER(rt_denoise_params)]
     L.rt_obj_load.argtypes = [C.c_char_p, C.POINTER(C.POINTER(rt_obj))]
     L.rt_obj_destroy.argtypes = [C.POINTER(rt_obj)]
     L.rt_scene_save.argtypes = [vp, C.c_char_p]
@@ -446,10 +453,23 @@ class Image:
         return img
 
     @classmethod
-    def from_scene(cls, scene, width, height):
+    def from_scene(cls, scene, width, height, aov_pass=0):
+        """The scene's last path-traced frame (aov_pass 0), or its last AOV normal / albedo pass."""
         img = cls(width, height)
-        check(lib().rt_image_from_scene(img.h, scene.h))
+        if aov_pass:
+            check(lib().rt_image_from_scene_pass(img.h, scene.h, aov_pass))
+        else:
+            check(lib().rt_image_from_scene(img.h, scene.h))
         return img
+
+    def denoise(self, albedo=None, normal=None, iterations=3, sigma_color=0.12, sigma_normal=0.1, sigma_albedo=0.1,
+                hdr=True):
+        """A new Image: rt_denoise of this one with optional guide images."""
+        out = Image(self.width, self.height)
+        p = rt_denoise_params(iterations, sigma_color, sigma_normal, sigma_albedo, 1 if hdr else 0)
+        check(lib().rt_denoise(self.h, albedo.h if albedo else None, normal.h if normal else None, out.h,
+                               C.byref(p)))
+        return out
 
     def close(self):
         if getattr(self, "h", None):

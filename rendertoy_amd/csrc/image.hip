@@ -188,6 +188,57 @@ __global__ void export_kernel(const float *img, uint8_t *out, int W, int H, int 
     }
 }
 
+// Edge-avoiding a-trous wavelet filter (Dammertz, Sewtz, Hanika, Lensch,
+// "Edge-Avoiding A-Trous Wavelet Transform for fast Global Illumination
+// Filtering", HPG 2010): iteration k convolves with the 5x5 B3-spline kernel
+// dilated by 2^k, each tap weighted by exp(-|dc|^2 / sc^2) exp(-|dn|^2 / sn^2)
+// exp(-|da|^2 / sa^2) of the colour and of the guide images; sc halves every
+// iteration.  Colour distances use c / (1 + c) when hdr is set.  Taps outside
+// the image are skipped.
+__global__ void atrous_kernel(const float *src, float *dst, const float *albedo, const float *normal, int W, int H,
+                              int step, float inv_sc2, float inv_sn2, float inv_sa2, int hdr) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= H || j >= W) return;
+    const float h[5] = {1.0f / 16.0f, 1.0f / 4.0f, 3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
+    const int64_t p = (int64_t)i * W + j;
+    const float3 cp = ld3f(src, p);
+    const float3 tp = hdr ? make_float3(cp.x / (1.0f + cp.x), cp.y / (1.0f + cp.y), cp.z / (1.0f + cp.z)) : cp;
+    const float3 np = normal ? ld3f(normal, p) : make_float3(0.0f, 0.0f, 0.0f);
+    const float3 ap = albedo ? ld3f(albedo, p) : make_float3(0.0f, 0.0f, 0.0f);
+    float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
+    for (int dy = -2; dy <= 2; ++dy) {
+        const int y = i + dy * step;
+        if (y < 0 || y >= H) continue;
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int x = j + dx * step;
+            if (x < 0 || x >= W) continue;
+            const int64_t q = (int64_t)y * W + x;
+            const float3 cq = ld3f(src, q);
+            const float3 tq = hdr ? make_float3(cq.x / (1.0f + cq.x), cq.y / (1.0f + cq.y), cq.z / (1.0f + cq.z)) : cq;
+            float e = ((tp.x - tq.x) * (tp.x - tq.x) + (tp.y - tq.y) * (tp.y - tq.y) + (tp.z - tq.z) * (tp.z - tq.z)) *
+                      inv_sc2;
+            if (normal) {
+                const float3 nq = ld3f(normal, q);
+                e += ((np.x - nq.x) * (np.x - nq.x) + (np.y - nq.y) * (np.y - nq.y) + (np.z - nq.z) * (np.z - nq.z)) *
+                     inv_sn2;
+            }
+            if (albedo) {
+                const float3 aq = ld3f(albedo, q);
+                e += ((ap.x - aq.x) * (ap.x - aq.x) + (ap.y - aq.y) * (ap.y - aq.y) + (ap.z - aq.z) * (ap.z - aq.z)) *
+                     inv_sa2;
+            }
+            const float w = h[dx + 2] * h[dy + 2] * expf(-e);
+            acc.x += w * cq.x;
+            acc.y += w * cq.y;
+            acc.z += w * cq.z;
+            wsum += w;
+        }
+    }
+    st3f(dst, p, make_float3(acc.x / wsum, acc.y / wsum, acc.z / wsum));
+}
+
 struct HipErr : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
@@ -490,6 +541,59 @@ int rt_image_arith(rt_image *dst, const rt_image *a, const rt_image *b, int op) 
         chk(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
         pixel(dst, kAdd + op, a->d, b->d, dst->d, 0.0f, 0);
         sync(dst);
+    });
+}
+
+// Denoiser interface: the filter OpenImageDenoise's "RT" filter stands for in
+// demo/WithOIDN/main.cpp:59-70 (beauty, optional albedo and normal guides,
+// output, hdr), built in as the edge-avoiding a-trous filter above.
+int rt_denoise(const rt_image *color, const rt_image *albedo, const rt_image *normal, rt_image *output,
+               const rt_denoise_params *prm) {
+    return guarded([&]() {
+        check_img(color);
+        check_img(output);
+        const uint32_t W = color->width, H = color->height;
+        auto same = [&](const rt_image *x) { return !x || (x->width == W && x->height == H); };
+        if (!same(albedo) || !same(normal) || !same(output)) throw BadArg("denoiser images differ in size");
+        rt_denoise_params p{3, 0.12f, 0.1f, 0.1f, 1};
+        if (prm) p = *prm;
+        if (p.iterations == 0 || p.iterations > 12) throw BadArg("iterations must be in [1, 12]");
+        if (!(p.sigma_color > 0.0f) || !(p.sigma_normal > 0.0f) || !(p.sigma_albedo > 0.0f))
+            throw BadArg("sigmas must be positive");
+        for (const rt_image *x : {color, albedo, normal})
+            if (x) chk(hipStreamSynchronize(x->stream), "hipStreamSynchronize");
+        // ping-pong between the output image and its scratch buffer, ending in the output
+        const size_t bytes = (size_t)W * H * 12;
+        float *bufs[2] = {output->d, output->tmp};
+        float *cur = bufs[p.iterations & 1u];
+        chk(hipMemcpyAsync(cur, color->d, bytes, hipMemcpyDeviceToDevice, output->stream), "hipMemcpyAsync");
+        dim3 grid((W + 63) / 64, (H + 3) / 4);
+        float sc = p.sigma_color;
+        for (uint32_t k = 0; k < p.iterations; ++k) {
+            float *nxt = (cur == bufs[0]) ? bufs[1] : bufs[0];
+            hipLaunchKernelGGL(atrous_kernel, grid, dim3(256), 0, output->stream, cur, nxt,
+                               albedo ? albedo->d : nullptr, normal ? normal->d : nullptr, (int)W, (int)H, 1 << k,
+                               1.0f / (sc * sc), 1.0f / (p.sigma_normal * p.sigma_normal),
+                               1.0f / (p.sigma_albedo * p.sigma_albedo), p.hdr ? 1 : 0);
+            chk(hipGetLastError(), "atrous_kernel");
+            cur = nxt;
+            sc *= 0.5f;
+        }
+        sync(output);
+    });
+}
+
+int rt_image_from_scene_pass(rt_image *img, rt_scene *scene, uint32_t pass) {
+    return guarded([&]() {
+        check_img(img);
+        if (pass != 0 && pass != RT_AOV_NORMAL && pass != RT_AOV_ALBEDO)
+            throw BadArg("pass must be 0 (path-traced frame), RT_AOV_NORMAL or RT_AOV_ALBEDO");
+        void *p = nullptr;
+        size_t bytes = 0;
+        if (rt_scene_device_output(scene, pass, &p, &bytes) != RT_OK) throw BadArg(rt_last_error());
+        if (bytes != (size_t)npix(img) * 12) throw BadArg("image size differs from the scene's pass");
+        chk(hipMemcpyAsync(img->d, p, bytes, hipMemcpyDeviceToDevice, img->stream), "hipMemcpyAsync");
+        sync(img);
     });
 }
 

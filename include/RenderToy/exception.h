@@ -14,6 +14,7 @@ class IRenderToyException : public std::runtime_error {
 class ImageSizeNotMatchException : public IRenderToyException {
   public:
     ImageSizeNotMatchException() : IRenderToyException("image sizes do not match") {}
+    explicit ImageSizeNotMatchException(const std::string &what) : IRenderToyException(what) {}
 };
 // A failure reported by the device library (rt_last_error()).
 class DeviceException : public IRenderToyException {

@@ -1,4 +1,6 @@
-// RenderToy/exporter.h -- image writers (include/RenderToy/exporter.h in RenderToy).
+// RenderToy/exporter.h -- image writers (include/RenderToy/exporter.h in RenderToy),
+// written through rt_image_export: the same bytes as RenderToy's exporters,
+// with BMP rows padded correctly for widths that are not a multiple of 4.
 #ifndef RENDERTOY_AMD_EXPORTER_H
 #define RENDERTOY_AMD_EXPORTER_H
 
@@ -8,22 +10,26 @@
 
 namespace RenderToy {
 
-// 24-bit BMP, rows bottom-up (buffer row 0 first), each channel int(255.99*clamp(v,0,1)).
-class BMPExporter {
-    const Image &img;
-
-  public:
-    explicit BMPExporter(const Image &img_) : img(img_) {}
-    void Export(std::ostream &os) const;
+struct IImageExporter {
+    const Image &image;
+    virtual void Export(std::ostream &os) = 0;
+    IImageExporter(const Image &image_) : image(image_) {}
+    virtual ~IImageExporter() = default;
 };
 
-// Binary PPM (P6), rows top-down.
-class PPMExporter {
-    const Image &img;
+struct PPMExporter : public IImageExporter {
+    virtual void Export(std::ostream &os) override final;
+    PPMExporter(const Image &image_) : IImageExporter(image_) {}
+};
 
-  public:
-    explicit PPMExporter(const Image &img_) : img(img_) {}
-    void Export(std::ostream &os) const;
+struct BMPExporter : public IImageExporter {
+    virtual void Export(std::ostream &os) override final;
+    BMPExporter(const Image &image_) : IImageExporter(image_) {}
+};
+
+struct ASCIIExporter : public IImageExporter {
+    virtual void Export(std::ostream &os) override final;
+    ASCIIExporter(const Image &image_) : IImageExporter(image_) {}
 };
 
 }  // namespace RenderToy

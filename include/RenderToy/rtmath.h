@@ -18,6 +18,7 @@
 #include <istream>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 namespace RenderToy {
 
@@ -216,6 +217,8 @@ struct PointN {
     PointN() : x(0), y(0) {}
     PointN(int x_, int y_) : x(x_), y(y_) {}
     PointN operator+(const PointN &p) const { return PointN(x + p.x, y + p.y); }
+    // clamps to [0, width] x [0, height] (inclusive), as RenderToy's PointN::SizeClamp
+    void SizeClamp(const SizeN &size);
 };
 
 struct SizeN {
@@ -227,6 +230,82 @@ struct SizeN {
     PointN Center() const { return PointN(static_cast<int>(width) >> 1, static_cast<int>(height) >> 1); }
     bool operator==(const SizeN &a) const { return width == a.width && height == a.height; }
     bool operator!=(const SizeN &a) const { return !(*this == a); }
+};
+
+inline void PointN::SizeClamp(const SizeN &size) {
+    x = std::clamp(x, 0, static_cast<int>(size.width));
+    y = std::clamp(y, 0, static_cast<int>(size.height));
+}
+
+// A rows x columns matrix of weights, indexed [row][column].
+template <typename T>
+struct GeneralizedMatrix {
+    std::size_t row, column;
+    std::vector<std::vector<T>> data;
+    GeneralizedMatrix(std::size_t r = 2, std::size_t c = 2, T v = T(0)) : row(r), column(c), data(r, std::vector<T>(c, v)) {}
+    GeneralizedMatrix(std::initializer_list<std::vector<T>> rows)
+        : row(rows.size()), column(rows.begin()->size()), data(rows) {}
+    std::vector<T> &operator[](std::size_t r) { return data[r]; }
+    const std::vector<T> &operator[](std::size_t r) const { return data[r]; }
+};
+
+// Convolution kernels of Image::Convolute (RenderToy rtmath.h:975-1090).
+template <typename T>
+struct IConvolutionKernel {
+    IConvolutionKernel() = default;
+    IConvolutionKernel(std::size_t r, std::size_t c) : kernel_mat(r, c) {}
+    IConvolutionKernel(const GeneralizedMatrix<T> &m) : kernel_mat(m) {}
+    GeneralizedMatrix<T> kernel_mat;
+};
+
+// exp(-d^2 / (2 sigma sigma)) normalised to sum 1: size x size (All), 1 x size (X)
+// or size x 1 (Y), centred at size / 2, in float with pow/exp as RenderToy.
+template <typename T, Orientation O = Orientation::All>
+struct UnitizedGaussianKernel : public IConvolutionKernel<T> {
+    UnitizedGaussianKernel(const std::size_t size, const T sigma)
+        : IConvolutionKernel<T>(O == Orientation::X ? 1 : size, O == Orientation::Y ? 1 : size) {
+        const T orig = T(size >> 1);
+        T sum = T(0);
+        auto &m = this->kernel_mat;
+        if constexpr (O == Orientation::All) {
+            for (std::size_t i = 0; i < size; ++i) {
+                const T y2 = std::pow(T(i) - orig, T(2));
+                for (std::size_t j = 0; j < size; ++j) {
+                    const T x2 = std::pow(T(j) - orig, T(2));
+                    const T g = std::exp(-(x2 + y2) / (T(2) * sigma * sigma));
+                    sum += g;
+                    m[i][j] = g;
+                }
+            }
+        } else {
+            for (std::size_t j = 0; j < size; ++j) {
+                const T d2 = std::pow(T(j) - orig, T(2));
+                const T g = std::exp(-(d2) / (T(2) * sigma * sigma));
+                sum += g;
+                (O == Orientation::X ? m[0][j] : m[j][0]) = g;
+            }
+        }
+        for (auto &r : m.data)
+            for (auto &v : r) v /= sum;
+    }
+};
+
+template <typename T, Orientation O>
+struct SobelKernel : public IConvolutionKernel<T> {
+    SobelKernel()
+        : IConvolutionKernel<T>(O == Orientation::X ? GeneralizedMatrix<T>({{T(-1), T(0), T(1)}, {T(-2), T(0), T(2)},
+                                                                             {T(-1), T(0), T(1)}})
+                                                    : GeneralizedMatrix<T>({{T(1), T(2), T(1)}, {T(0), T(0), T(0)},
+                                                                             {T(-1), T(-2), T(-1)}})) {}
+};
+
+template <typename T, Orientation O>
+struct PrewittKernel : public IConvolutionKernel<T> {
+    PrewittKernel()
+        : IConvolutionKernel<T>(O == Orientation::X ? GeneralizedMatrix<T>({{T(-1), T(0), T(1)}, {T(-1), T(0), T(1)},
+                                                                             {T(-1), T(0), T(1)}})
+                                                    : GeneralizedMatrix<T>({{T(1), T(1), T(1)}, {T(0), T(0), T(0)},
+                                                                             {T(-1), T(-1), T(-1)}})) {}
 };
 
 // x*(1-a) + y*a

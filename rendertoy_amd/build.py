@@ -62,11 +62,11 @@ def build(verbose: bool = True) -> str:
 
 
 def build_reference_demos(reference: str, verbose: bool = True) -> list:
-    """RenderToy's own demo programs (demo/CornellBox, demo/SmoothShading), compiled
-    unchanged against include/RenderToy/ and linked to this library.  Needs the
-    RenderToy sources; the binaries land next to the library."""
+    """RenderToy's own demo programs (demo/CornellBox, demo/SmoothShading,
+    demo/Drawing), compiled unchanged against include/RenderToy/ and linked to
+    this library.  Needs the RenderToy sources; the binaries land next to the library."""
     built = []
-    for d in ("CornellBox", "SmoothShading"):
+    for d in ("CornellBox", "SmoothShading", "Drawing"):
         src = os.path.join(reference, "demo", d, "main.cpp")
         if not os.path.exists(src):
             continue

@@ -468,6 +468,27 @@ void OBJModelImporter::Import(World &world, const std::string &path) {
 
 // ---------------------------------------------------------------- Image
 
+namespace {
+// A host Image's pixels on the device for one rt_image_* call.
+struct DevImage {
+    rt_image *h = nullptr;
+    explicit DevImage(const Image &img) {
+        check(rt_image_create((uint32_t)img.resolution.width, (uint32_t)img.resolution.height, &h));
+        const int rc = rt_image_upload(h, reinterpret_cast<const float *>(img.GetBuffer()));
+        if (rc != RT_OK) {
+            rt_image_destroy(h);
+            check(rc);
+        }
+    }
+    void Download(Image &img) { check(rt_image_download(h, reinterpret_cast<float *>(img.GetBuffer()))); }
+    ~DevImage() { rt_image_destroy(h); }
+};
+
+void require_same(const SizeN &a, const SizeN &b, const char *what) {
+    if (a != b) throw Exception::ImageSizeNotMatchException(what);
+}
+}  // namespace
+
 Image::Image(const SizeN &res) : buffer(new Vector3f[res.Area()]), resolution(res) {}
 Image::Image(const RenderContext *const rc) : buffer(new Vector3f[rc->format_settings.resolution.Area()]),
                                               resolution(rc->format_settings.resolution) {
@@ -478,138 +499,147 @@ Image::Image(const Image &o) : buffer(new Vector3f[o.resolution.Area()]), resolu
 }
 Image::~Image() { delete[] buffer; }
 
-Image &Image::operator=(const Image &o) {
-    if (o.resolution != resolution) throw Exception::ImageSizeNotMatchException();
+const void Image::operator=(const Image &o) {
+    require_same(resolution, o.resolution, "The = operator of two images requires two images have the same resolution.");
     std::copy(o.buffer, o.buffer + resolution.Area(), buffer);
-    return *this;
 }
 
-Image Image::operator+(const Image &o) const {
-    if (o.resolution != resolution) throw Exception::ImageSizeNotMatchException();
-    Image r(resolution);
-    for (size_t i = 0; i < resolution.Area(); ++i) r.buffer[i] = buffer[i] + o.buffer[i];
-    return r;
-}
+#define RT_IMAGE_OP(OP, NAME)                                                                              \
+    const Image Image::operator OP(const Image &o) {                                                     \
+        require_same(resolution, o.resolution,                                                            \
+                     "The " #OP " operator of two images requires two images have the same resolution."); \
+        Image r(resolution);                                                                              \
+        for (size_t i = 0; i < resolution.Area(); ++i) r.buffer[i] = buffer[i] OP o.buffer[i];            \
+        return r;                                                                                         \
+    }
+RT_IMAGE_OP(+, add)
+RT_IMAGE_OP(-, sub)
+RT_IMAGE_OP(*, mul)
+RT_IMAGE_OP(/, div)
+#undef RT_IMAGE_OP
 
-Image Image::Extract(const std::function<bool(const Vector3f &)> &keep, const Vector3f &def) const {
+const Image Image::Extract(const std::function<bool(const Vector3f &)> &keep, const Vector3f &def) {
     Image r(resolution);
     for (size_t i = 0; i < resolution.Area(); ++i) r.buffer[i] = keep(buffer[i]) ? buffer[i] : def;
     return r;
 }
 
-void Image::Transform(const std::function<void(Vector3f &)> &f) {
+const void Image::Transform(const std::function<void(Vector3f &)> &f) {
     for (size_t i = 0; i < resolution.Area(); ++i) f(buffer[i]);
 }
 
+void Image::DeviceConvolute(const std::vector<float> &w, std::size_t rows, std::size_t cols, unsigned flags) {
+    DevImage d(*this);
+    check(rt_image_convolve(d.h, (uint32_t)rows, (uint32_t)cols, w.data(), flags));
+    d.Download(*this);
+}
+
+void Image::DeviceTonemap(float limit, int standard) {
+    DevImage d(*this);
+    check(rt_image_tonemap(d.h, limit, standard));
+    d.Download(*this);
+}
+
+void Image::DeviceGreyScale(int standard) {
+    DevImage d(*this);
+    check(rt_image_greyscale(d.h, standard));
+    d.Download(*this);
+}
+
+void Image::DeviceBinarize(int standard, float threshold) {
+    DevImage d(*this);
+    check(rt_image_binarize(d.h, standard, threshold));
+    d.Download(*this);
+}
+
+void Image::DeviceEdgeDetection(int orientation) {
+    DevImage d(*this);
+    check(rt_image_edge_detect(d.h, orientation));
+    d.Download(*this);
+}
+
 Image &Image::GaussianBlur(const std::size_t size, const float sigma) {
-    if (size == 0) return *this;
-    const long half = (long)size / 2;
-    std::vector<float> w(size);
-    float sum = 0.0f;
-    for (size_t i = 0; i < size; ++i) {
-        float d = float((long)i - half);
-        w[i] = std::exp(-d * d / (2.0f * sigma * sigma));
-        sum += w[i];
-    }
-    for (auto &x : w) x /= sum;
-    const long W = (long)resolution.width, H = (long)resolution.height;
-    auto reflect = [](long i, long n) {
-        if (n == 1) return 0L;
-        while (i < 0 || i >= n) i = i < 0 ? -i : 2 * (n - 1) - i;
-        return i;
-    };
-    std::vector<Vector3f> tmp(resolution.Area());
-    for (long y = 0; y < H; ++y)
-        for (long x = 0; x < W; ++x) {
-            Vector3f acc;
-            for (long k = 0; k < (long)size; ++k) acc += buffer[y * W + reflect(x + k - half, W)] * w[k];
-            tmp[y * W + x] = acc;
-        }
-    for (long y = 0; y < H; ++y)
-        for (long x = 0; x < W; ++x) {
-            Vector3f acc;
-            for (long k = 0; k < (long)size; ++k) acc += tmp[reflect(y + k - half, H) * W + x] * w[k];
-            buffer[y * W + x] = acc;
-        }
+    DevImage d(*this);
+    check(rt_image_gaussian_blur(d.h, (uint32_t)size, sigma));
+    d.Download(*this);
     return *this;
 }
 
 Image &Image::Bloom(const std::size_t size, const float sigma, const float threshold) {
-    Image layer = Extract([threshold](const Vector3f &c) { return Convert::Luma(c) > threshold; });
-    layer.GaussianBlur(size, sigma);
-    *this = *this + layer;
-    return *this;
-}
-
-Image &Image::Tonemap(const float limit) {
-    Transform([limit](Vector3f &c) { c = Convert::Tonemap(c, limit); });
+    DevImage d(*this);
+    check(rt_image_bloom(d.h, (uint32_t)size, sigma, threshold));
+    d.Download(*this);
     return *this;
 }
 
 Image &Image::GammaCorrection(const float gamma) {
-    const float e = 1.0f / gamma;
-    Transform([e](Vector3f &c) { c = Vector3f::Pow(c, Vector3f(e)); });
+    DevImage d(*this);
+    check(rt_image_gamma(d.h, gamma));
+    d.Download(*this);
     return *this;
+}
+
+// Writes go through the flat index y * width + x, kept inside the buffer: a
+// point clamped to x = width lands on the next row's first pixel, as in RenderToy.
+void Image::DrawLine(PointN p1, PointN p2, const Vector3f &color) {
+    p1.SizeClamp(resolution);
+    p2.SizeClamp(resolution);
+    const long W = (long)resolution.width, n = (long)resolution.Area();
+    auto put = [&](long y, long x) {
+        const long i = y * W + x;
+        if (i >= 0 && i < n) buffer[i] = color;
+    };
+    if (std::abs(p2.y - p1.y) < std::abs(p2.x - p1.x)) {   // x-major: step x, move y on error
+        if (p1.x > p2.x) std::swap(p1, p2);
+        const int ddy = std::abs(p2.y - p1.y), sy = p2.y - p1.y < 0 ? -1 : 1, ddx = p2.x - p1.x;
+        int err = 0, y = p1.y;
+        for (int x = p1.x; x < p2.x; ++x) {
+            if (err > ddx) {
+                y += sy;
+                err -= 2 * ddx;
+            }
+            put(y, x);
+            err += 2 * ddy;
+        }
+    } else {   // y-major
+        if (p1.y > p2.y) std::swap(p1, p2);
+        const int ddx = std::abs(p2.x - p1.x), sx = p2.x - p1.x < 0 ? -1 : 1, ddy = p2.y - p1.y;
+        int err = 0, x = p1.x;
+        for (int y = p1.y; y < p2.y; ++y) {
+            if (err > ddy) {
+                x += sx;
+                err -= 2 * ddy;
+            }
+            put(y, x);
+            err += 2 * ddx;
+        }
+    }
+}
+
+void Image::Fill(PointN p1, PointN p2, const Vector3f &color) {
+    p1.SizeClamp(resolution);
+    p2.SizeClamp(resolution);
+    if (p1.x > p2.x) std::swap(p1.x, p2.x);
+    if (p1.y > p2.y) std::swap(p1.y, p2.y);
+    for (int x = p1.x; x < p2.x; ++x)
+        for (int y = p1.y; y < p2.y; ++y) buffer[(size_t)y * resolution.width + x] = color;
 }
 
 // ---------------------------------------------------------------- exporters
 
 namespace {
-unsigned char to8(float v) {
-    v = std::min(1.0f, std::max(0.0f, v));
-    return (unsigned char)int(255.99f * v);
-}
-void put32(std::ostream &os, uint32_t v) {
-    for (int i = 0; i < 4; ++i) os.put(char((v >> (8 * i)) & 0xFF));
-}
-void put16(std::ostream &os, uint16_t v) {
-    os.put(char(v & 0xFF));
-    os.put(char(v >> 8));
+void export_with(const Image &img, int format, std::ostream &os) {
+    DevImage d(img);
+    size_t n = 0;
+    check(rt_image_export(d.h, format, nullptr, 0, &n));
+    std::vector<uint8_t> bytes(n);
+    check(rt_image_export(d.h, format, bytes.data(), bytes.size(), &n));
+    os.write(reinterpret_cast<const char *>(bytes.data()), (std::streamsize)n);
 }
 }  // namespace
 
-void BMPExporter::Export(std::ostream &os) const {
-    const uint32_t W = (uint32_t)img.resolution.width, H = (uint32_t)img.resolution.height;
-    const uint32_t row = (3 * W + 3) & ~3u, data = row * H;
-    os.put('B');
-    os.put('M');
-    put32(os, 54 + data);
-    put32(os, 0);
-    put32(os, 54);
-    put32(os, 40);
-    put32(os, W);
-    put32(os, H);
-    put16(os, 1);
-    put16(os, 24);
-    put32(os, 0);
-    put32(os, data);
-    put32(os, 2835);
-    put32(os, 2835);
-    put32(os, 0);
-    put32(os, 0);
-    const Vector3f *b = img.GetBuffer();
-    for (uint32_t y = 0; y < H; ++y) {
-        for (uint32_t x = 0; x < W; ++x) {
-            const Vector3f &c = b[(size_t)y * W + x];
-            os.put((char)to8(c.z()));
-            os.put((char)to8(c.y()));
-            os.put((char)to8(c.x()));
-        }
-        for (uint32_t p = 3 * W; p < row; ++p) os.put(0);
-    }
-}
-
-void PPMExporter::Export(std::ostream &os) const {
-    const size_t W = img.resolution.width, H = img.resolution.height;
-    os << "P6\n" << W << " " << H << "\n255\n";
-    const Vector3f *b = img.GetBuffer();
-    for (size_t y = H; y-- > 0;)
-        for (size_t x = 0; x < W; ++x) {
-            const Vector3f &c = b[y * W + x];
-            os.put((char)to8(c.x()));
-            os.put((char)to8(c.y()));
-            os.put((char)to8(c.z()));
-        }
-}
+void PPMExporter::Export(std::ostream &os) { export_with(image, RT_EXPORT_PPM, os); }
+void BMPExporter::Export(std::ostream &os) { export_with(image, RT_EXPORT_BMP, os); }
+void ASCIIExporter::Export(std::ostream &os) { export_with(image, RT_EXPORT_ASCII, os); }
 
 }  // namespace RenderToy

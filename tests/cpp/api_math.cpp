@@ -4,6 +4,7 @@
 //   S gate_x gate_y focal W H      -> 2 words: PrepareScreenSpace top, right
 //   O path                         -> triangles, meshes, emitters, then per triangle
 //                                     13 words (vert_w[0], edges, normal, area) and its mesh
+//   G size sigma orient            -> UnitizedGaussianKernel<float, All/X/Y> weights, row-major
 #include <RenderToy/rendertoy.h>
 
 #include <cstdint>
@@ -48,6 +49,18 @@ int main() {
             p.Screen(&cam, top, right);
             hex(top);
             hex(right);
+            std::cout << "\n";
+        } else if (cmd == "G") {
+            size_t size;
+            float sigma;
+            int orient;
+            std::cin >> size >> sigma >> orient;
+            IConvolutionKernel<float> k;
+            if (orient == 0) k = UnitizedGaussianKernel<float, Orientation::All>(size, sigma);
+            else if (orient == 1) k = UnitizedGaussianKernel<float, Orientation::X>(size, sigma);
+            else k = UnitizedGaussianKernel<float, Orientation::Y>(size, sigma);
+            for (size_t i = 0; i < k.kernel_mat.row; ++i)
+                for (size_t j = 0; j < k.kernel_mat.column; ++j) hex(k.kernel_mat[i][j]);
             std::cout << "\n";
         } else if (cmd == "O") {
             std::string path;

@@ -22,7 +22,10 @@ black, the value the oracle and the device define for them.
 OBJ fixtures (part `obj`) are what RenderToy's OBJModelImporter builds from
 OBJ texts written here (a scene of our own and quirk cases).
 
-Usage: python tests/golden/gen_golden.py [part ...]   (parts: geometry aov pt bsdf camera converged image obj)
+Demo fixtures (part `demos`): RenderToy's demo/Drawing built against RenderToy
+itself; only the hash and size of the image it writes are kept.
+
+Usage: python tests/golden/gen_golden.py [part ...]   (parts: geometry aov pt bsdf camera converged image obj demos)
 """
 from __future__ import annotations
 
@@ -343,8 +346,26 @@ def part_obj():
     np.savez_compressed(out("obj.npz"), **res)
 
 
+def part_demos():
+    """RenderToy's demo/Drawing compiled against RenderToy itself (oracle/_ref)
+    and run: the sha256 and size of the Drawing.bmp it writes."""
+    import hashlib
+    import tempfile
+    refroot = os.environ.get("REF_ROOT", "/root/reference")
+    d = tempfile.mkdtemp()
+    exe = os.path.join(d, "drawing")
+    subprocess.run(["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-include", "optional",
+                    "-I" + os.path.join(refroot, "include"), os.path.join(refroot, "demo", "Drawing", "main.cpp"),
+                    os.path.join(ROOT, "oracle", "_ref", "libref.so"), "-fopenmp",
+                    "-Wl,-rpath," + os.path.join(ROOT, "oracle", "_ref"), "-o", exe], check=True)
+    subprocess.run([exe], cwd=d, check=True)
+    data = open(os.path.join(d, "Drawing.bmp"), "rb").read()
+    np.savez_compressed(out("demos.npz"), drawing_sha256=np.array(hashlib.sha256(data).hexdigest()),
+                        drawing_bytes=np.array(len(data)))
+
+
 PARTS = {"geometry": part_geometry, "aov": part_aov, "pt": part_pt, "bsdf": part_bsdf, "camera": part_camera,
-         "converged": part_converged, "image": part_image, "obj": part_obj}
+         "converged": part_converged, "image": part_image, "obj": part_obj, "demos": part_demos}
 
 if __name__ == "__main__":
     a = sys.argv[1:]

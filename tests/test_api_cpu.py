@@ -74,7 +74,17 @@ def test_obj_import_and_triangle_cache(probe, tmp_path, name):
         assert int(w[13]) == sc.tri_mesh[i]
 
 
+def test_gaussian_kernel_matches_reference(probe):
+    """UnitizedGaussianKernel of the C++ API: RenderToy's weights bit for bit (tests/golden/image.npz)."""
+    gold = np.load(os.path.join(GOLD, "image.npz"))
+    for orient in range(3):
+        w = probe("G 9 2.5 %d\n" % orient).split()
+        assert np.array_equal(np.array([int(x, 16) for x in w], np.uint32), bits(gold["gauss9_%d" % orient]))
+    w = probe("G 500 50 1\n").split()
+    assert np.array_equal(np.array([int(x, 16) for x in w], np.uint32), bits(gold["gauss500_x"]))
+
+
 @pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "demo")), reason="RenderToy sources not present")
-@pytest.mark.parametrize("demo", ["CornellBox", "SmoothShading"])
+@pytest.mark.parametrize("demo", ["CornellBox", "SmoothShading", "Drawing"])
 def test_reference_demo_compiles_unchanged(tmp_path, demo):
     subprocess.run(["g++", *FLAGS, "-fsyntax-only", os.path.join(REFERENCE, "demo", demo, "main.cpp")], check=True)

@@ -81,3 +81,18 @@ def test_reference_smoothshading_demo_runs(tmp_path):
     assert r.returncode == 0, r.stderr
     w, h, mean = _bmp_mean(tmp_path / "ss.bmp")
     assert (w, h) == (1920, 1080) and mean > 5
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(LIB, "demo_Drawing")), reason="reference demos not built")
+def test_reference_drawing_demo_is_byte_identical(tmp_path):
+    """RenderToy's demo/Drawing (DrawLine, Fill, Binarize<BT601>, BMPExporter),
+    compiled unchanged against include/RenderToy/, writes the same Drawing.bmp
+    as when compiled against RenderToy itself (tests/golden/demos.npz)."""
+    import hashlib
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "demos.npz"))
+    r = subprocess.run([os.path.join(LIB, "demo_Drawing")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    data = (tmp_path / "Drawing.bmp").read_bytes()
+    assert len(data) == int(gold["drawing_bytes"])
+    assert hashlib.sha256(data).hexdigest() == str(gold["drawing_sha256"])

@@ -4,8 +4,9 @@
 // (include/RenderToy/rtmath.h:126-150,320-329,399-418) and the library is
 // compiled with -ffp-contract=off, so ray generation, Moller-Trumbore and the
 // shading formulas round exactly as the reference's IEEE single precision
-// code does.  Box culling is the only place that uses its own arithmetic; it
-// is padded so it can only keep extra boxes, never drop a hit.
+// code does.  Box culling is the only place that uses its own arithmetic (with
+// fused multiply-adds); it is padded so it can only keep extra boxes, never
+// drop a hit.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -128,7 +129,7 @@ struct Hit {
 // Traversal stack: the top kLdsStack entries of every lane live in LDS
 // (entry-major, stride = threads per block), deeper entries spill to a global
 // region (entry-major, stride = threads in the grid).
-constexpr int kLdsStack = 16;
+constexpr int kLdsStack = 12;
 
 struct Stack {
     uint32_t *lref;
@@ -166,10 +167,15 @@ __device__ __forceinline__ float safe_rcp(float d) {
     return 1.0f / dd;
 }
 
-// Relative slack on box distances; rounding of (lo - o) * (1/d) is at most a
-// few ulp, far below this.
-constexpr float kSlackLo = 1.0f - 1.0f / 262144.0f;
-constexpr float kSlackHi = 1.0f + 1.0f / 262144.0f;
+// Box distances are fmaf(plane, 1/d, -RN(o/d)): one fused operation per
+// plane.  With u = 2^-24, the error against (plane - o) / d is at most
+// u|o/d| + u|t| <= u|plane/d| + 2u|t| (|o| <= |plane| + |plane - o|).  Boxes
+// are padded outward by 2^-19 |plane| (32u |plane|, bvh_build.cpp), which
+// absorbs the first term, and a box is kept while entry <= exit * kSlack and
+// entry <= best * kSlack, a relative slack of 2^-17 (128u) that absorbs the
+// second with a wide margin.  So the culling can keep extra boxes, never drop
+// one a ray enters.
+constexpr float kSlack = 1.0f + 1.0f / 131072.0f;
 
 #define RT_CSWAP(a, b)                                                      \
     if (tn[b] < tn[a]) {                                                    \
@@ -185,6 +191,7 @@ template <bool ANY, bool COUNT>
 __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t ex0, uint32_t ex1, float tmax,
                                       Stack &stk, Hit &hit, uint32_t &n_nodes, uint32_t &n_tris) {
     const float ix = safe_rcp(d.x), iy = safe_rcp(d.y), iz = safe_rcp(d.z);
+    const float ox = o.x * ix, oy = o.y * iy, oz = o.z * iz;
     float best = ANY ? tmax : kFloatInfinity;
     uint32_t best_id = kNone;
     float bu = 0.0f, bv = 0.0f;
@@ -196,7 +203,7 @@ __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t e
             float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
             uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
             if (COUNT) ++n_nodes;
-            float lim = best * kSlackHi;
+            const float lim = best * kSlack;
             float tn[4];
             uint32_t rf[4] = {ch.x, ch.y, ch.z, ch.w};
             const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
@@ -204,12 +211,12 @@ __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t e
             const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                float t0x = (lxa[j] - o.x) * ix, t1x = (hxa[j] - o.x) * ix;
-                float t0y = (lya[j] - o.y) * iy, t1y = (hya[j] - o.y) * iy;
-                float t0z = (lza[j] - o.z) * iz, t1z = (hza[j] - o.z) * iz;
-                float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
-                float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-                bool h = rf[j] != kEmptyRef && a * kSlackLo <= b * kSlackHi && a * kSlackLo <= lim;
+                const float t0x = fmaf(lxa[j], ix, -ox), t1x = fmaf(hxa[j], ix, -ox);
+                const float t0y = fmaf(lya[j], iy, -oy), t1y = fmaf(hya[j], iy, -oy);
+                const float t0z = fmaf(lza[j], iz, -oz), t1z = fmaf(hza[j], iz, -oz);
+                const float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+                const float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+                const bool h = rf[j] != kEmptyRef && a <= b * kSlack && a <= lim;
                 tn[j] = h ? a : __builtin_inff();
             }
             RT_CSWAP(0, 1);
@@ -248,12 +255,12 @@ __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t e
     pop:
         {
             bool found = false;
-            float lim = best * kSlackHi;
+            const float lim = best * kSlack;
             while (stk.sp > 0) {
                 uint32_t r;
                 float t;
                 stk.pop(r, t);
-                if (t * kSlackLo <= lim) {
+                if (t <= lim) {
                     cur = r;
                     found = true;
                     break;

@@ -13,6 +13,7 @@
 // reads black.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -29,6 +30,7 @@ struct rt_image {
     uint32_t width = 0, height = 0;
     float *d = nullptr;        // the image
     float *tmp = nullptr;      // scratch image of the same size (convolution source, bloom layer)
+    float *tmp2 = nullptr;     // second scratch image, allocated by the first bloom
     float *k = nullptr;        // convolution weights
     size_t k_cap = 0;          // floats the weight buffer holds
     ~rt_image();
@@ -38,6 +40,7 @@ namespace rtamd {
 namespace {
 
 constexpr uint32_t kLdsTaps = 8192;   // weights kept in LDS up to this many taps
+constexpr uint32_t kMaxGridY = 65535;  // kernels loop over rows beyond this many y blocks
 
 __device__ __forceinline__ float3 ld3f(const float *p, int64_t i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 __device__ __forceinline__ void st3f(float *p, int64_t i, float3 v) {
@@ -46,12 +49,13 @@ __device__ __forceinline__ void st3f(float *p, int64_t i, float3 v) {
     p[3 * i + 2] = v.z;
 }
 
-__device__ __forceinline__ float3 reflect_read(const float *img, int W, int H, int px, int py) {
+__device__ __forceinline__ float3 reflect_read(const float *img, int W, int H, int x, int y) {
+    int64_t px = x, py = y;
     if (px < 0) px = -px;
     if (py < 0) py = -py;
-    if (py >= W) py = 2 * W - py;
-    if (px >= H) px = 2 * H - px;
-    const int64_t flat = (int64_t)px * W + py;
+    if (py >= W) py = 2 * (int64_t)W - py;
+    if (px >= H) px = 2 * (int64_t)H - px;
+    const int64_t flat = px * W + py;
     if (flat < 0 || flat >= (int64_t)W * H) return make_float3(0.0f, 0.0f, 0.0f);
     return ld3f(img, flat);
 }
@@ -69,6 +73,20 @@ __device__ __forceinline__ float luma(float3 v, int standard) {
 
 __device__ __forceinline__ float length3(float3 v) { return sqrtf(0.0f + v.x * v.x + v.y * v.y + v.z * v.z); }
 
+// Convolute's post-processing switches, in the reference's order.
+__device__ __forceinline__ float3 conv_finish(float3 acc, int flags) {
+    if (flags & RT_CONV_ABS) acc = make_float3(fabsf(acc.x), fabsf(acc.y), fabsf(acc.z));
+    if (flags & RT_CONV_LENGTH) {
+        const float l = length3(acc);
+        acc = make_float3(l, l, l);
+    }
+    if (flags & RT_CONV_NORMALIZE) {
+        const float l = length3(acc);
+        acc = make_float3(acc.x / l, acc.y / l, acc.z / l);
+    }
+    return acc;
+}
+
 // Image::Convolute<_Normalize, _Length, _Abs> (compositor.h:107-146).  Block
 // 64 x 4 threads: a wave covers 64 consecutive columns of one row, so every
 // tap is one coalesced read.
@@ -82,29 +100,129 @@ __global__ __launch_bounds__(256) void conv_kernel(const float *src, float *dst,
     __syncthreads();
     const float *kk = in_lds ? s_k : k;
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= H || j >= W) return;
     const int r2 = rows >> 1, c2 = cols >> 1;
-    float3 acc = make_float3(0.0f, 0.0f, 0.0f);
-    for (int ii = 0; ii < rows; ++ii)
-        for (int jj = 0; jj < cols; ++jj) {
-            const float3 v = reflect_read(src, W, H, i - r2 + ii, j - c2 + jj);
-            const float w = kk[ii * cols + jj];
-            acc.x = acc.x + v.x * w;
-            acc.y = acc.y + v.y * w;
-            acc.z = acc.z + v.z * w;
-        }
-    if (flags & RT_CONV_ABS) acc = make_float3(fabsf(acc.x), fabsf(acc.y), fabsf(acc.z));
-    if (flags & RT_CONV_LENGTH) {
-        const float l = length3(acc);
-        acc = make_float3(l, l, l);
+    for (int i = blockIdx.y * 4 + (threadIdx.x >> 6); i < H; i += gridDim.y * 4) {
+        if (j >= W) break;
+        float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+        for (int ii = 0; ii < rows; ++ii)
+            for (int jj = 0; jj < cols; ++jj) {
+                const float3 v = reflect_read(src, W, H, i - r2 + ii, j - c2 + jj);
+                const float w = kk[ii * cols + jj];
+                acc.x = acc.x + v.x * w;
+                acc.y = acc.y + v.y * w;
+                acc.z = acc.z + v.z * w;
+            }
+        st3f(dst, (int64_t)i * W + j, conv_finish(acc, flags));
     }
-    if (flags & RT_CONV_NORMALIZE) {
-        const float l = length3(acc);
-        acc = make_float3(acc.x / l, acc.y / l, acc.z / l);
-    }
-    st3f(dst, (int64_t)i * W + j, acc);
 }
+
+// One-row convolution (rows == 1: the X pass of GaussianBlur and Bloom).  A
+// block computes 256 consecutive pixels of one row; it first stages the
+// 256 + cols - 1 source pixels those taps read (edge reads resolved as in
+// reflect_read) into LDS as three colour planes, then every thread sums its
+// taps from LDS in the reference's order.  Weights are read with a
+// wave-uniform index (scalar loads).
+constexpr int kRowBlock = 256;
+constexpr uint32_t kRowMaxTaps = 4096;   // LDS: 3 x (256 + 4095) floats = 51 KB
+
+__global__ __launch_bounds__(256) void conv_row_kernel(const float *__restrict__ src, float *__restrict__ dst, int W,
+                                                       int H, int cols, const float *__restrict__ k, int flags) {
+    extern __shared__ float s_seg[];
+    const int n = kRowBlock + cols - 1;
+    const int j0 = blockIdx.x * kRowBlock;
+    const int j = j0 + (int)threadIdx.x;
+    const int c2 = cols >> 1;
+    const float *sx = s_seg + threadIdx.x, *sy = sx + n, *sz = sy + n;
+    for (int i = blockIdx.y; i < H; i += gridDim.y) {   // block-uniform
+        for (int e = threadIdx.x; e < n; e += kRowBlock) {
+            const float3 v = reflect_read(src, W, H, i, j0 - c2 + e);
+            s_seg[e] = v.x;
+            s_seg[n + e] = v.y;
+            s_seg[2 * n + e] = v.z;
+        }
+        __syncthreads();
+        if (j < W) {
+            float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+            for (int jj = 0; jj < cols; ++jj) {
+                const float w = k[jj];
+                acc.x = acc.x + sx[jj] * w;
+                acc.y = acc.y + sy[jj] * w;
+                acc.z = acc.z + sz[jj] * w;
+            }
+            st3f(dst, (int64_t)i * W + j, conv_finish(acc, flags));
+        }
+        __syncthreads();   // the segment is rewritten for the next row
+    }
+}
+
+// One-column convolution (cols == 1: the Y pass).  A wave covers 64
+// consecutive columns and M consecutive output rows; source row p is loaded
+// once (one coalesced read per lane) and added into every output row m whose
+// tap p - (row_m - rows/2) lies in [0, rows).  Each output still receives its
+// taps in increasing order, one IEEE multiply and add at a time.
+template <int M>
+__device__ __forceinline__ void col_add(float3 (&acc)[M], float3 v, const float *__restrict__ k, int s, int rows) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int t = s - m;
+        if (t >= 0 && t < rows) {
+            const float w = k[t];
+            acc[m].x = acc[m].x + v.x * w;
+            acc[m].y = acc[m].y + v.y * w;
+            acc[m].z = acc[m].z + v.z * w;
+        }
+    }
+}
+
+// Source row p of column j as reflect_read(src, W, H, p, j) reads it: with
+// 0 <= j < W the flat index is inside the buffer exactly when the reflected
+// row is in [0, H), a wave-uniform test, so the load itself is unconditional
+// (row 0 stands in for a black row) and the zero is selected afterwards.
+__device__ __forceinline__ float3 col_fetch(const float *__restrict__ colp, int64_t W, int64_t H, int64_t p) {
+    if (p < 0) p = -p;
+    if (p >= H) p = 2 * H - p;
+    const bool ok = p >= 0 && p < H;
+    const float *q = colp + 3 * W * (ok ? p : 0);
+    const float3 v = make_float3(q[0], q[1], q[2]);
+    return ok ? v : make_float3(0.0f, 0.0f, 0.0f);
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void conv_col_kernel(const float *__restrict__ src, float *__restrict__ dst, int W,
+                                                       int H, int rows, const float *__restrict__ k, int flags) {
+    const int j = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    if (j >= W) return;   // no barriers below
+    const int r2 = rows >> 1;
+    const float *colp = src + 3 * (int64_t)j;
+    // first output row of this wave, wave-uniform
+    for (int ib = (int)(blockIdx.y * 4 + (threadIdx.x >> 6)) * M; ib < H; ib += (int)gridDim.y * 4 * M) {
+        const int64_t p0 = (int64_t)ib - r2;
+        float3 acc[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) acc[m] = make_float3(0.0f, 0.0f, 0.0f);
+        int s = 0;
+        if (rows >= M) {
+            for (; s < M - 1; ++s) col_add<M>(acc, col_fetch(colp, W, H, p0 + s), k, s, rows);
+            // every output row takes this source row: no bounds on the tap index
+#pragma unroll 4
+            for (; s < rows; ++s) {
+                const float3 v = col_fetch(colp, W, H, p0 + s);
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    const float w = k[s - m];
+                    acc[m].x = acc[m].x + v.x * w;
+                    acc[m].y = acc[m].y + v.y * w;
+                    acc[m].z = acc[m].z + v.z * w;
+                }
+            }
+        }
+        for (; s < rows + M - 1; ++s) col_add<M>(acc, col_fetch(colp, W, H, p0 + s), k, s, rows);
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (ib + m < H) st3f(dst, (int64_t)(ib + m) * W + j, conv_finish(acc[m], flags));
+    }
+}
+constexpr int kColRows = 4;   // output rows per thread in the column pass
 
 // Per-pixel operations (compositor.h:184-228, compositor.cpp:70-101).
 enum PixOp { kExtract = 0, kTonemap, kGamma, kGrey, kBinarize, kEdgeCombine, kAdd, kSub, kMul, kDiv };
@@ -281,10 +399,24 @@ void convolve(rt_image *img, const float *src, float *dst, uint32_t rows, uint32
         img->k_cap = n;
     }
     chk(hipMemcpyAsync(img->k, w, n * sizeof(float), hipMemcpyHostToDevice, img->stream), "hipMemcpyAsync");
-    dim3 grid((img->width + 63) / 64, (img->height + 3) / 4);
-    hipLaunchKernelGGL(conv_kernel, grid, dim3(256), 0, img->stream, src, dst, (int)img->width, (int)img->height,
-                       (int)rows, (int)cols, img->k, flags);
-    chk(hipGetLastError(), "conv_kernel");
+    const int W = (int)img->width, H = (int)img->height;
+    if (rows == 1 && cols <= kRowMaxTaps) {
+        const size_t lds = 3 * (size_t)(kRowBlock + cols - 1) * sizeof(float);
+        dim3 grid((img->width + kRowBlock - 1) / kRowBlock, std::min(img->height, kMaxGridY));
+        hipLaunchKernelGGL(conv_row_kernel, grid, dim3(kRowBlock), lds, img->stream, src, dst, W, H, (int)cols, img->k,
+                           flags);
+        chk(hipGetLastError(), "conv_row_kernel");
+    } else if (cols == 1) {
+        dim3 grid((img->width + 63) / 64, std::min((img->height + 4 * kColRows - 1) / (4 * kColRows), kMaxGridY));
+        hipLaunchKernelGGL(conv_col_kernel<kColRows>, grid, dim3(256), 0, img->stream, src, dst, W, H, (int)rows,
+                           img->k, flags);
+        chk(hipGetLastError(), "conv_col_kernel");
+    } else {
+        dim3 grid((img->width + 63) / 64, std::min((img->height + 3) / 4, kMaxGridY));
+        hipLaunchKernelGGL(conv_kernel, grid, dim3(256), 0, img->stream, src, dst, W, H, (int)rows, (int)cols, img->k,
+                           flags);
+        chk(hipGetLastError(), "conv_kernel");
+    }
     // the host weights must outlive the asynchronous copy
     sync(img);
 }
@@ -322,11 +454,8 @@ std::vector<float> gaussian_1d(uint32_t size, float sigma) {
 // GaussianBlur (compositor.cpp:42-49) of the image buffer `buf` through `tmp`.
 void gaussian_blur(rt_image *img, float *buf, float *scratch, uint32_t size, float sigma) {
     const std::vector<float> w = gaussian_1d(size, sigma);
-    const size_t bytes = (size_t)npix(img) * 12;
-    chk(hipMemcpyAsync(scratch, buf, bytes, hipMemcpyDeviceToDevice, img->stream), "hipMemcpyAsync");
-    convolve(img, scratch, buf, 1, size, w.data(), 0);
-    chk(hipMemcpyAsync(scratch, buf, bytes, hipMemcpyDeviceToDevice, img->stream), "hipMemcpyAsync");
-    convolve(img, scratch, buf, size, 1, w.data(), 0);
+    convolve(img, buf, scratch, 1, size, w.data(), 0);   // X pass
+    convolve(img, scratch, buf, size, 1, w.data(), 0);   // Y pass
 }
 
 void check_img(const rt_image *img) {
@@ -342,6 +471,7 @@ rt_image::~rt_image() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (d) (void)hipFree(d);
     if (tmp) (void)hipFree(tmp);
+    if (tmp2) (void)hipFree(tmp2);
     if (k) (void)hipFree(k);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -447,26 +577,12 @@ int rt_image_bloom(rt_image *img, uint32_t size, float sigma, float threshold) {
     return guarded([&]() {
         check_img(img);
         if (size == 0) throw BadArg("kernel size must be >= 1");
-        float *layer = nullptr, *scratch = nullptr;
-        const size_t bytes = (size_t)npix(img) * 12;
-        chk(hipMalloc(&layer, bytes), "hipMalloc");
-        hipError_t e = hipMalloc(&scratch, bytes);
-        if (e != hipSuccess) {
-            (void)hipFree(layer);
-            chk(e, "hipMalloc");
-        }
-        try {
-            pixel(img, kExtract, img->d, nullptr, layer, threshold, 0);
-            gaussian_blur(img, layer, scratch, size, sigma);
-            pixel(img, kAdd, img->d, layer, img->d, 0.0f, 0);
-            sync(img);
-        } catch (...) {
-            (void)hipFree(layer);
-            (void)hipFree(scratch);
-            throw;
-        }
-        (void)hipFree(layer);
-        (void)hipFree(scratch);
+        if (!img->tmp2) chk(hipMalloc(&img->tmp2, (size_t)npix(img) * 12), "hipMalloc");
+        // layer = the bright pixels, blurred through tmp2, added back
+        pixel(img, kExtract, img->d, nullptr, img->tmp, threshold, 0);
+        gaussian_blur(img, img->tmp, img->tmp2, size, sigma);
+        pixel(img, kAdd, img->d, img->tmp, img->d, 0.0f, 0);
+        sync(img);
     });
 }
 

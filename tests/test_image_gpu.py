@@ -149,3 +149,45 @@ def test_device_bmp_pads_rows(gold):
     px = np.frombuffer(bmp[54:], np.uint8).reshape(H, row)
     assert np.all(px[:, 3 * W:] == 0)
     assert np.array_equal(px[:, :3 * W].reshape(H, W, 3)[:, :, ::-1], rgb)
+
+
+def same_bits_or_nan(a, b):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return np.array_equal(np.where(both_nan, 0, a).view(np.uint32), np.where(both_nan, 0, b).view(np.uint32))
+
+
+@pytest.mark.parametrize("W,H", [(300, 70), (517, 3), (5, 40), (1, 1), (2, 2), (9, 4)])
+def test_device_separable_convolutions_match_oracle(W, H):
+    """1 x K and K x 1 kernels (the row and column convolution kernels) against
+    the oracle, bit for bit: kernels narrower and wider than the image, even
+    and odd lengths, every Convolute switch."""
+    from py_oracle import Oracle
+    from rendertoy_amd import capi
+    rng = np.random.default_rng(W * 1000 + H)
+    img = rng.uniform(0.01, 2.0, (H, W, 3)).astype(np.float32)
+    o = Oracle()
+    for K in (1, 2, 3, 4, 5, 17, 64, 301):
+        for shape in ((1, K), (K, 1)):
+            k = rng.uniform(-0.5, 1.0, shape).astype(np.float32)
+            for f in (0, 3, 4, 7):
+                got = capi.Image.from_array(img).convolve(k, f).array()
+                assert same_bits_or_nan(got, o.image_convolve(img, k, f)), (W, H, shape, f)
+    # wider than the row kernel's LDS segment: the general kernel takes it
+    if (W, H) == (9, 4):
+        k = rng.uniform(-0.5, 1.0, (1, 4100)).astype(np.float32)
+        assert same_bits_or_nan(capi.Image.from_array(img).convolve(k, 0).array(), o.image_convolve(img, k, 0))
+
+
+@pytest.mark.parametrize("W,H,shape", [(2, 70000, (1, 3)), (1, 1100000, (3, 1)), (1, 300000, (3, 3))])
+def test_device_convolution_tall_images(W, H, shape):
+    """Images taller than one launch's y blocks cover (65,535 rows of blocks):
+    every convolution kernel loops over the remaining rows."""
+    from py_oracle import Oracle
+    from rendertoy_amd import capi
+    rng = np.random.default_rng(H)
+    img = rng.uniform(0.0, 1.0, (H, W, 3)).astype(np.float32)
+    k = rng.uniform(-0.5, 1.0, shape).astype(np.float32)
+    got = capi.Image.from_array(img).convolve(k, 0).array()
+    assert bits_equal(got, Oracle().image_convolve(img, k, 0))

@@ -117,7 +117,7 @@ def bench_c3(n_frames=2):
         rays += b.stats()["rays"]
     dt = (time.perf_counter() - t0) / n_frames
     for s in ss:
-        for _ in range(2):
+        for _ in range(3):   # the tuner's trials
             s.render_pt(spp=32, max_depth=6, download=False)
     t0 = time.perf_counter()
     for s in ss:
@@ -425,7 +425,7 @@ def main():
         del s0
         sc2 = scenes.terrain_c2()
         s2 = capi.Scene(sc2)
-        for _ in range(2):   # the launch-policy tuner times its two candidates on the first two frames
+        for _ in range(3):   # the launch-policy tuner times its three candidates on the first three frames
             s2.render_pt(download=False)
         t0 = time.perf_counter()
         n2 = 3

@@ -15,7 +15,7 @@
 namespace rtamd {
 
 template <bool COUNT>
-__global__ __launch_bounds__(256) void aov_kernel(DevScene sc, rt_camera cam, AovDev out, uint32_t passes,
+RT_TRACE_KERNEL void aov_kernel(DevScene sc, rt_camera cam, AovDev out, uint32_t passes,
                                                   uint32_t *counters, uint32_t tiles_x, uint32_t ntiles,
                                                   uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride,
                                                   unsigned long long *stat) {
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void aov_kernel(DevScene sc, rt_camera cam, Ao
 }
 
 // BVH::Intersect over an explicit ray batch; a wave's work item is 64 rays.
-__global__ __launch_bounds__(256) void ray_kernel(DevScene sc, const float *src, const float *dir,
+RT_TRACE_KERNEL void ray_kernel(DevScene sc, const float *src, const float *dir,
                                                   const uint32_t *exclude, float4 *out, uint32_t n,
                                                   uint32_t *fetch, uint32_t *ovf_ref, float *ovf_t,
                                                   int64_t ovf_stride) {

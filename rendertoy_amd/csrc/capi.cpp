@@ -120,6 +120,7 @@ void validate_camera(const rt_camera *cam) {
     if (!cam) throw InvalidArg{"camera is NULL"};
     if (cam->width == 0 || cam->height == 0) throw InvalidArg{"camera has zero width or height"};
     if ((uint64_t)cam->width * cam->height >= (1ull << 31)) throw InvalidArg{"frame larger than 2^31 pixels"};
+    if (frame_tiles(cam->width, cam->height) * 64u >= (1ull << 32)) throw InvalidArg{"frame holds 2^26 or more 8x8 tiles"};
 }
 
 }  // namespace
@@ -144,6 +145,7 @@ rt_scene *make_render_context(int device) {
         for (auto &e : s->ev) RT_HIP(hipEventCreate(&e));
         s->ds = DevScene{};
         s->ds.share_rays = s->ds.share_prim = kSchedAuto;
+        s->ds.refill = kRayRefill;
     } catch (...) {
         delete s;
         throw;
@@ -298,6 +300,7 @@ int rt_scene_create(const rt_scene_desc *desc, rt_scene **out) {
             ds.ground[k] = desc->ground_reflection[k];
         }
         ds.share_rays = ds.share_prim = kSchedAuto;
+        ds.refill = kRayRefill;
         s->num_tris = n;
         s->num_materials = desc->num_materials;
         s->bvh_nodes = (uint32_t)bvh.nodes.size();
@@ -465,6 +468,7 @@ int rt_scene_load(const char *path, rt_scene **out) {
             ds.ground[k] = h.ground[k];
         }
         ds.share_rays = ds.share_prim = kSchedAuto;
+        ds.refill = kRayRefill;
         for (int k = 0; k < 6; ++k) s->bbox[k] = h.bbox[k];
         s->num_tris = h.num_tris;
         s->num_materials = h.num_materials;

@@ -9,6 +9,7 @@
 // drop a hit.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/rendertoy_amd.h"
@@ -119,7 +120,12 @@ struct DevScene {
     // band's chunks for explicit ray batches and path-tracer primary tiles
     // (see BandSched; the trace queues' share is in PtDev)
     uint32_t share_rays, share_prim;
+    // idle lanes of a wave that trigger a refill in the AOV and ray kernels (trace_pool)
+    uint32_t refill;
 };
+
+// Default refill threshold of the traversal kernels (trace_pool).
+constexpr uint32_t kRayRefill = 16u;
 
 struct Hit {
     uint32_t id;
@@ -130,6 +136,11 @@ struct Hit {
 // (entry-major, stride = threads per block), deeper entries spill to a global
 // region (entry-major, stride = threads in the grid).
 constexpr int kLdsStack = 12;
+// Waves per SIMD that the LDS stack of a 256-lane block allows (160 KB of LDS
+// per CU, 4 SIMDs): traversal kernels ask the compiler to fit their registers
+// to this occupancy.
+constexpr int kTraceWavesPerSimd = (160 * 1024) / (kLdsStack * 8 * 256);
+#define RT_TRACE_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd)))
 
 struct Stack {
     uint32_t *lref;
@@ -183,10 +194,144 @@ constexpr float kSlack = 1.0f + 1.0f / 131072.0f;
         uint32_t _r = rf[a]; rf[a] = rf[b]; rf[b] = _r;                     \
     }
 
+// Triangles of leaf `ref` against the ray (Moller-Trumbore, closest hit by
+// (t, triangle index)).  ANY: returns true, with the occluder in best_id, as
+// soon as one is hit before `best` (the distance bound).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool leaf_test(const float4 *tris, uint32_t ref, V3 o, V3 d, uint32_t ex0, uint32_t ex1,
+                                          float &best, uint32_t &best_id, float &bu, float &bv, uint32_t &n_tris) {
+    const uint32_t start = (ref >> 3) & 0x0FFFFFFFu;
+    const uint32_t cnt = (ref & 7u) + 1u;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const float4 *tp = tris + 3 * (size_t)(start + k);
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        const uint32_t id = __float_as_uint(c.y);
+        if (COUNT) ++n_tris;
+        if (id == ex0 || (ANY && id == ex1)) continue;
+        float t, u, v;
+        if (mt_intersect(o, d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v)) {
+            if (ANY) {
+                if (t < best) {
+                    best_id = id;
+                    return true;
+                }
+            } else if (t < kFloatInfinity && (t < best || (t == best && id < best_id))) {
+                best = t;
+                best_id = id;
+                bu = u;
+                bv = v;
+            }
+        }
+    }
+    return false;
+}
+
+// Traversal state of one ray, advanced one BVH4 node or leaf per step.
+struct RayState {
+    float ix, iy, iz;        // 1 / direction
+    float ox, oy, oz;        // origin / direction, rounded
+    uint32_t near_off;       // byte offsets of the near x, y, z planes in a Node4, 8 bits each
+    float best, bu, bv;      // closest hit so far (ANY: the distance bound)
+    uint32_t best_id;        // its triangle, kNone if none (ANY: the occluder once found)
+    uint32_t cur;            // node or leaf to visit next
+    uint32_t ex0, ex1;       // excluded triangles
+    V3 d;                    // direction (Moller-Trumbore)
+    V3 o;                    // origin
+};
+
+template <bool ANY>
+__device__ __forceinline__ void ray_init(RayState &r, V3 o, V3 d, uint32_t ex0, uint32_t ex1, float tmax,
+                                         Stack &stk) {
+    r.ix = safe_rcp(d.x);
+    r.iy = safe_rcp(d.y);
+    r.iz = safe_rcp(d.z);
+    r.ox = o.x * r.ix;
+    r.oy = o.y * r.iy;
+    r.oz = o.z * r.iz;
+    // fmaf(plane, inv, -o*inv) is monotonic in the plane, so the near plane's
+    // distance is the smaller of the two: the same values a min/max over both
+    // planes would give, with no min/max.
+    r.near_off = (r.ix < 0.0f ? 16u : 0u) | (r.iy < 0.0f ? 48u : 32u) << 8 | (r.iz < 0.0f ? 80u : 64u) << 16;
+    r.best = ANY ? tmax : kFloatInfinity;
+    r.bu = 0.0f;
+    r.bv = 0.0f;
+    r.best_id = kNone;
+    r.cur = 0;
+    r.ex0 = ex0;
+    r.ex1 = ex1;
+    r.d = d;
+    r.o = o;
+    stk.sp = 0;
+}
+
+// One step: the node or leaf r.cur, then (after a leaf, or a node none of
+// whose children is entered) the next stack entry still in range.  Returns
+// true when the ray is finished: stack exhausted, or (ANY) an occluder found,
+// left in r.best_id.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool trace_step(const Node4 *nodes, const float4 *tris, RayState &r, Stack &stk,
+                                           uint32_t &n_nodes, uint32_t &n_tris) {
+    if (!(r.cur & kLeafBit)) {
+        const char *np = reinterpret_cast<const char *>(nodes + r.cur);
+        const uint32_t onx = r.near_off & 0xFFu, ony = (r.near_off >> 8) & 0xFFu, onz = r.near_off >> 16;
+        const float4 nx = *reinterpret_cast<const float4 *>(np + onx);
+        const float4 fx = *reinterpret_cast<const float4 *>(np + (onx ^ 16u));
+        const float4 ny = *reinterpret_cast<const float4 *>(np + ony);
+        const float4 fy = *reinterpret_cast<const float4 *>(np + (ony ^ 16u));
+        const float4 nz = *reinterpret_cast<const float4 *>(np + onz);
+        const float4 fz = *reinterpret_cast<const float4 *>(np + (onz ^ 16u));
+        const uint4 ch = *reinterpret_cast<const uint4 *>(np + offsetof(Node4, child));
+        if (COUNT) ++n_nodes;
+        const float lim = r.best * kSlack;
+        float tn[4];
+        uint32_t rf[4] = {ch.x, ch.y, ch.z, ch.w};
+        const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+        const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+        const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float a = fmaxf(fmaxf(fmaf(nxa[j], r.ix, -r.ox), fmaf(nya[j], r.iy, -r.oy)),
+                                  fmaxf(fmaf(nza[j], r.iz, -r.oz), 0.0f));
+            const float b = fminf(fminf(fmaf(fxa[j], r.ix, -r.ox), fmaf(fya[j], r.iy, -r.oy)),
+                                  fmaf(fza[j], r.iz, -r.oz));
+            const bool h = rf[j] != kEmptyRef && a <= b * kSlack && a <= lim;
+            tn[j] = h ? a : __builtin_inff();
+        }
+        RT_CSWAP(0, 1);
+        RT_CSWAP(2, 3);
+        RT_CSWAP(0, 2);
+        RT_CSWAP(1, 3);
+        RT_CSWAP(1, 2);
+        if (tn[0] != __builtin_inff()) {
+            if (tn[3] != __builtin_inff()) stk.push(rf[3], tn[3]);
+            if (tn[2] != __builtin_inff()) stk.push(rf[2], tn[2]);
+            if (tn[1] != __builtin_inff()) stk.push(rf[1], tn[1]);
+            r.cur = rf[0];
+            return false;
+        }
+    } else if (leaf_test<ANY, COUNT>(tris, r.cur, r.o, r.d, r.ex0, r.ex1, r.best, r.best_id, r.bu, r.bv, n_tris)) {
+        return true;
+    }
+    const float lim = r.best * kSlack;
+    while (stk.sp > 0) {
+        uint32_t ref;
+        float t;
+        stk.pop(ref, t);
+        if (t <= lim) {
+            r.cur = ref;
+            return false;
+        }
+    }
+    return true;
+}
+
 // Closest hit (ANY = false): minimum (t, triangle index) over triangles other
 // than `ex0` with t < 1e7 -- the result of BVH::Intersect's brute-force form
 // (src/bvh.cpp:268-286).  Any hit (ANY = true): returns true as soon as a
-// triangle other than ex0/ex1 is hit with t < tmax.
+// triangle other than ex0/ex1 is hit with t < tmax.  The result does not
+// depend on the visiting order.  One ray per lane from start to end; the slab
+// test takes min/max over both planes of each axis (measured faster here than
+// trace_step's near/far plane offsets, which win in trace_pool).
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t ex0, uint32_t ex1, float tmax,
                                       Stack &stk, Hit &hit, uint32_t &n_nodes, uint32_t &n_tris) {
@@ -200,8 +345,8 @@ __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t e
     while (true) {
         if (!(cur & kLeafBit)) {
             const float4 *np = reinterpret_cast<const float4 *>(sc.nodes + cur);
-            float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
             if (COUNT) ++n_nodes;
             const float lim = best * kSlack;
             float tn[4];
@@ -224,50 +369,29 @@ __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t e
             RT_CSWAP(0, 2);
             RT_CSWAP(1, 3);
             RT_CSWAP(1, 2);
-            if (tn[0] == __builtin_inff()) goto pop;
-            if (tn[3] != __builtin_inff()) stk.push(rf[3], tn[3]);
-            if (tn[2] != __builtin_inff()) stk.push(rf[2], tn[2]);
-            if (tn[1] != __builtin_inff()) stk.push(rf[1], tn[1]);
-            cur = rf[0];
-            continue;
-        } else {
-            uint32_t start = (cur >> 3) & 0x0FFFFFFFu;
-            uint32_t cnt = (cur & 7u) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 *tp = sc.tris + 3 * (size_t)(start + k);
-                float4 a = tp[0], b = tp[1], c = tp[2];
-                uint32_t id = __float_as_uint(c.y);
-                if (COUNT) ++n_tris;
-                if (id == ex0 || (ANY && id == ex1)) continue;
-                float t, u, v;
-                if (mt_intersect(o, d, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, V3{b.z, b.w, c.x}, t, u, v)) {
-                    if (ANY) {
-                        if (t < tmax) return true;
-                    } else if (t < kFloatInfinity && (t < best || (t == best && id < best_id))) {
-                        best = t;
-                        best_id = id;
-                        bu = u;
-                        bv = v;
-                    }
-                }
+            if (tn[0] != __builtin_inff()) {
+                if (tn[3] != __builtin_inff()) stk.push(rf[3], tn[3]);
+                if (tn[2] != __builtin_inff()) stk.push(rf[2], tn[2]);
+                if (tn[1] != __builtin_inff()) stk.push(rf[1], tn[1]);
+                cur = rf[0];
+                continue;
+            }
+        } else if (leaf_test<ANY, COUNT>(sc.tris, cur, o, d, ex0, ex1, best, best_id, bu, bv, n_tris)) {
+            return true;
+        }
+        bool found = false;
+        const float lim = best * kSlack;
+        while (stk.sp > 0) {
+            uint32_t r;
+            float t;
+            stk.pop(r, t);
+            if (t <= lim) {
+                cur = r;
+                found = true;
+                break;
             }
         }
-    pop:
-        {
-            bool found = false;
-            const float lim = best * kSlack;
-            while (stk.sp > 0) {
-                uint32_t r;
-                float t;
-                stk.pop(r, t);
-                if (t <= lim) {
-                    cur = r;
-                    found = true;
-                    break;
-                }
-            }
-            if (!found) break;
-        }
+        if (!found) break;
     }
     hit.id = best_id;
     hit.t = best;
@@ -369,6 +493,77 @@ __device__ __forceinline__ uint32_t band_next(BandSched &s) {
         ++s.kb;
     }
     return 0xFFFFFFFFu;
+}
+
+// Rays [0, count), dealt in chunks of 64 by a BandSched, traced by one wave
+// with lane refill.  A lane whose ray is finished idles until `refill` lanes
+// of the wave are idle; then the finished rays go to `finish` together and the
+// idle lanes take the next rays of the current chunk, or of a new one.  With
+// refill = 64 a wave traces one chunk to the end before it takes the next (as
+// slow as the chunk's longest ray); lower values keep more lanes busy at the
+// price of mixing rays of neighbouring chunks.  Results do not depend on it.
+//   fetch(j, r, nodes, tris, tag) -> bool: ray_init ray j into r (BATCH: also
+//     set this lane's scene arrays); false if ray j is void.  tag: any value
+//     the caller wants back in finish.
+//   finish(j, r, tag): consume the finished ray.
+// Call with every lane of the wave (no divergence): it uses ballots.
+template <bool ANY, bool COUNT, bool BATCH, class Fetch, class Finish>
+__device__ __forceinline__ void trace_pool(BandSched &bs, uint32_t count, uint32_t refill, const Node4 *nodes,
+                                           const float4 *tris, Stack &stk, Fetch &&fetch, Finish &&finish,
+                                           uint32_t &n_nodes, uint32_t &n_tris) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    bool active = false, pending = false;
+    uint32_t j = 0, tag = 0;
+    RayState r;
+    const Node4 *ln = nodes;
+    const float4 *lt = tris;
+    uint32_t pool_next = 0, pool_end = 0;   // unclaimed rays of the current chunk (wave-uniform)
+    bool sched_done = false;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (n_idle >= refill || n_idle == 64u) {
+            if (pending) {
+                finish(j, r, tag);
+                pending = false;
+            }
+            const uint32_t a0 = pool_next, n0 = pool_end - pool_next;
+            uint32_t a1 = 0, n1 = 0;
+            if (n0 < n_idle && !sched_done) {
+                const uint32_t c = band_next(bs);
+                if (c == 0xFFFFFFFFu) {
+                    sched_done = true;
+                } else {
+                    a1 = c * 64u;
+                    n1 = count - a1 < 64u ? count - a1 : 64u;
+                }
+            }
+            if (!active) {
+                const uint32_t rk = (uint32_t)__popcll(idle & below);
+                uint32_t jj = 0xFFFFFFFFu;
+                if (rk < n0) jj = a0 + rk;
+                else if (rk - n0 < n1) jj = a1 + (rk - n0);
+                if (jj != 0xFFFFFFFFu) {
+                    j = jj;
+                    active = fetch(j, r, ln, lt, tag);
+                }
+            }
+            const uint32_t u0 = n_idle < n0 ? n_idle : n0;
+            if (n1 != 0u) {
+                const uint32_t u1 = n_idle - u0 < n1 ? n_idle - u0 : n1;
+                pool_next = a1 + u1;
+                pool_end = a1 + n1;
+            } else {
+                pool_next = a0 + u0;
+            }
+            if (__ballot(active) == 0ull && sched_done && pool_next >= pool_end) break;
+        }
+        if (active && trace_step<ANY, COUNT>(BATCH ? ln : nodes, BATCH ? lt : tris, r, stk, n_nodes, n_tris)) {
+            active = false;
+            pending = true;
+        }
+    }
 }
 
 }  // namespace rtamd

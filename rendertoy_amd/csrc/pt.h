@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
 
 struct rt_scene;
 
@@ -10,5 +11,8 @@ struct PtState;
 void pt_destroy(PtState *p);
 int pt_device_output(rt_scene *s, void **dev_ptr, size_t *bytes);
 uint32_t persistent_grid(rt_scene *s, int blocks_per_cu);
+// 8x8 tiles of a frame; primary-ray kernels index rays as tile * 64 + pixel in
+// 32 bits, so frames need tiles * 64 < 2^32.
+inline uint64_t frame_tiles(uint32_t w, uint32_t h) { return (uint64_t)((w + 7u) / 8u) * ((h + 7u) / 8u); }
 void ensure_overflow(rt_scene *s, uint32_t threads);
 }  // namespace rtamd

@@ -82,7 +82,7 @@ __device__ __forceinline__ uint32_t find_range(const uint32_t *off, uint32_t n, 
 }
 
 template <bool BATCH>
-__global__ __launch_bounds__(256) void pt_primary_kernel(DevScene sc, PtDev p, uint32_t tiles_x, uint32_t ntiles,
+RT_TRACE_KERNEL void pt_primary_kernel(DevScene sc, PtDev p, uint32_t tiles_x, uint32_t ntiles,
                                                          uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
     __shared__ uint32_t s_ref[kLdsStack * 256];
     __shared__ float s_t[kLdsStack * 256];
@@ -302,9 +302,12 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
     }
 }
 
-template <bool SHADOW, bool BATCH>
-__global__ __launch_bounds__(256) void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t depth,
-                                                       uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
+// Extension (SHADOW = false) or shadow rays of the queue for (batch, depth).
+// POOL: lane refill through trace_pool (divergent secondary rays keep more
+// lanes busy); otherwise each wave traces its chunk of 64 rays to the end.
+template <bool SHADOW, bool BATCH, bool POOL>
+RT_TRACE_KERNEL void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t depth, uint32_t *ovf_ref,
+                                     float *ovf_t, int64_t ovf_stride) {
     __shared__ uint32_t s_ref[kLdsStack * 256];
     __shared__ float s_t[kLdsStack * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
@@ -315,30 +318,64 @@ __global__ __launch_bounds__(256) void pt_trace_kernel(DevScene sc, PtDev p, uin
     uint32_t nn = 0, nt = 0;
     BandSched bs = band_sched((count + 63u) / 64u, p.fctr + (depth * 2u + (SHADOW ? 1u : 0u)) * 8u * 32u,
                               p.share_trace);
-    while (true) {
-        const uint32_t chunk = band_next(bs);
-        if (chunk == kNone) break;
-        const uint32_t j = chunk * 64u + lane;
-        if (j >= count) continue;
-        const uint32_t slot = q[j];
-        const DevScene &S = BATCH ? p.bscn[p.sid[slot]].sc : sc;
-        const float4 ro = p.ray_o[slot];
-        const float4 rd = p.ray_d[slot];
-        const V3 o = {ro.x, ro.y, ro.z};
-        const uint32_t ex = __float_as_uint(rd.w);
-        Hit h;
-        if (SHADOW) {
-            const float4 sd = p.sh_d[slot];
-            const float4 c = p.sh_c[slot];
-            const bool occluded = trace<true, false>(S, o, V3{sd.x, sd.y, sd.z}, ex, __float_as_uint(c.w), sd.w,
-                                                     stk, h, nn, nt);
-            if (!occluded) {
-                float4 l = p.rad[slot];
-                p.rad[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+    if constexpr (POOL) {
+        auto fetch = [&](uint32_t j, RayState &r, const Node4 *&ln, const float4 *&lt, uint32_t &slot) {
+            slot = q[j];
+            if (BATCH) {
+                const DevScene &S = p.bscn[p.sid[slot]].sc;
+                ln = S.nodes;
+                lt = S.tris;
             }
-        } else {
-            trace<false, false>(S, o, V3{rd.x, rd.y, rd.z}, ex, kNone, 0.0f, stk, h, nn, nt);
-            p.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+            const float4 ro = p.ray_o[slot];
+            const float4 rd = p.ray_d[slot];
+            const V3 o = {ro.x, ro.y, ro.z};
+            if (SHADOW) {
+                const float4 sd = p.sh_d[slot];
+                ray_init<true>(r, o, V3{sd.x, sd.y, sd.z}, __float_as_uint(rd.w), __float_as_uint(p.sh_c[slot].w),
+                               sd.w, stk);
+            } else {
+                ray_init<false>(r, o, V3{rd.x, rd.y, rd.z}, __float_as_uint(rd.w), kNone, 0.0f, stk);
+            }
+            return true;
+        };
+        auto finish = [&](uint32_t, const RayState &r, uint32_t slot) {
+            if (SHADOW) {
+                if (r.best_id == kNone) {   // not occluded
+                    const float4 c = p.sh_c[slot];
+                    const float4 l = p.rad[slot];
+                    p.rad[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+                }
+            } else {
+                p.hit[slot] = make_float4(r.best, r.bu, r.bv, __uint_as_float(r.best_id));
+            }
+        };
+        trace_pool<SHADOW, false, BATCH>(bs, count, p.refill, sc.nodes, sc.tris, stk, fetch, finish, nn, nt);
+    } else {
+        while (true) {
+            const uint32_t chunk = band_next(bs);
+            if (chunk == kNone) break;
+            const uint32_t j = chunk * 64u + lane;
+            if (j >= count) continue;
+            const uint32_t slot = q[j];
+            const DevScene &S = BATCH ? p.bscn[p.sid[slot]].sc : sc;
+            const float4 ro = p.ray_o[slot];
+            const float4 rd = p.ray_d[slot];
+            const V3 o = {ro.x, ro.y, ro.z};
+            const uint32_t ex = __float_as_uint(rd.w);
+            Hit h;
+            if (SHADOW) {
+                const float4 sd = p.sh_d[slot];
+                const float4 c = p.sh_c[slot];
+                const bool occluded = trace<true, false>(S, o, V3{sd.x, sd.y, sd.z}, ex, __float_as_uint(c.w), sd.w,
+                                                         stk, h, nn, nt);
+                if (!occluded) {
+                    const float4 l = p.rad[slot];
+                    p.rad[slot] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+                }
+            } else {
+                trace<false, false>(S, o, V3{rd.x, rd.y, rd.z}, ex, kNone, 0.0f, stk, h, nn, nt);
+                p.hit[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.id));
+            }
         }
     }
 }
@@ -376,9 +413,9 @@ __global__ void pt_zero_u32_kernel(uint32_t *p, size_t n) {
 
 int pt_trace_blocks_per_cu() {
     int nb = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false, false>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false, false, true>, 256, 0);
     int nb2 = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true, true>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true, true, true>, 256, 0);
     int nb3 = 0;
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb3, pt_primary_kernel<true>, 256, 0);
     nb = nb < nb2 ? nb : nb2;
@@ -418,8 +455,12 @@ static void enqueue_bounce_loop(const DevScene &sc, const PtDev &p, uint32_t bat
             hipLaunchKernelGGL((pt_shade_kernel<true, BATCH>), dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0,
                                npaths, 0u);
         } else {
-            hipLaunchKernelGGL((pt_trace_kernel<false, BATCH>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d,
-                               ovf_ref, ovf_t, ovf_stride);
+            if (p.refill < 64u)
+                hipLaunchKernelGGL((pt_trace_kernel<false, BATCH, true>), dim3(trace_grid), dim3(256), 0, s, sc, p,
+                                   batch, d, ovf_ref, ovf_t, ovf_stride);
+            else
+                hipLaunchKernelGGL((pt_trace_kernel<false, BATCH, false>), dim3(trace_grid), dim3(256), 0, s, sc, p,
+                                   batch, d, ovf_ref, ovf_t, ovf_stride);
             hipLaunchKernelGGL((pt_shade_kernel<false, BATCH>), dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0,
                                npaths, d);
         }
@@ -430,9 +471,12 @@ static void enqueue_bounce_loop(const DevScene &sc, const PtDev &p, uint32_t bat
             if (err != hipSuccess) return;
             break;
         }
-        if (shadows)
-            hipLaunchKernelGGL((pt_trace_kernel<true, BATCH>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch, d,
-                               ovf_ref, ovf_t, ovf_stride);
+        if (shadows && p.refill < 64u)
+            hipLaunchKernelGGL((pt_trace_kernel<true, BATCH, true>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch,
+                               d, ovf_ref, ovf_t, ovf_stride);
+        else if (shadows)
+            hipLaunchKernelGGL((pt_trace_kernel<true, BATCH, false>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch,
+                               d, ovf_ref, ovf_t, ovf_stride);
     }
 }
 

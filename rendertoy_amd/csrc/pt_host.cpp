@@ -41,10 +41,14 @@ struct PtState {
     float tune_ms[4] = {0, 0, 0, 0};
 };
 
-// Trace-queue policies tried by the tuner: all-static for small queues (no
-// atomics; best when ray costs are uniform), and all-dynamic (best when they vary).
-static const uint32_t kTraceShares[] = {kSchedAuto, 0u};
-constexpr uint32_t kNumTraceShares = sizeof(kTraceShares) / sizeof(kTraceShares[0]);
+// Trace-queue policies tried by the tuner: chunks all static for small queues
+// (no atomics; best when ray costs are uniform), all dynamic (best when they
+// vary), and dynamic with lane refill (trace_pool; best for divergent rays).
+struct TracePolicy {
+    uint32_t share, refill;   // refill 64: no refill, a wave traces its chunk to the end
+};
+static const TracePolicy kTracePolicies[] = {{kSchedAuto, 64u}, {0u, 64u}, {0u, kRayRefill}};
+constexpr uint32_t kNumTraceShares = sizeof(kTracePolicies) / sizeof(kTracePolicies[0]);
 
 namespace {
 
@@ -121,7 +125,8 @@ static int render_frame(rt_scene *s, const rt_camera *cam, rt_batch *batch, cons
             set_error("scene, camera or params is NULL");
             return RT_ERR_INVALID;
         }
-        if (cam && (cam->width == 0 || cam->height == 0 || (uint64_t)cam->width * cam->height >= (1ull << 31))) {
+        if (cam && (cam->width == 0 || cam->height == 0 || (uint64_t)cam->width * cam->height >= (1ull << 31) ||
+                    frame_tiles(cam->width, cam->height) * 64u >= (1ull << 32))) {
             set_error("bad frame size");
             return RT_ERR_INVALID;
         }
@@ -187,7 +192,8 @@ static int render_frame(rt_scene *s, const rt_camera *cam, rt_batch *batch, cons
         // trace-queue policy: fixed, or the tuner's trial / choice for this workload shape
         uint32_t trial = kNumTraceShares;
         if (prm->flags & RT_PT_NO_TUNE) {
-            p.share_trace = kSchedAuto;
+            p.share_trace = kTracePolicies[0].share;
+            p.refill = kTracePolicies[0].refill;
         } else {
             const uint64_t tk[4] = {npix, ((uint64_t)prm->spp << 32) | prm->max_depth, (uint64_t)(uintptr_t)cache,
                                     ((uint64_t)K << 32) | (prm->flags & ~(uint32_t)RT_PT_NO_GRAPH)};
@@ -196,7 +202,9 @@ static int render_frame(rt_scene *s, const rt_camera *cam, rt_batch *batch, cons
                 st->tune_trial = 0;
             }
             if (st->tune_trial < kNumTraceShares) trial = st->tune_trial;
-            p.share_trace = kTraceShares[trial < kNumTraceShares ? trial : st->tune_best];
+            const TracePolicy &tp = kTracePolicies[trial < kNumTraceShares ? trial : st->tune_best];
+            p.share_trace = tp.share;
+            p.refill = tp.refill;
         }
         if (batch) {
             std::memset(&p.cam, 0, sizeof(p.cam));
@@ -452,8 +460,8 @@ int rt_batch_render(rt_batch *b, const rt_camera *cams, const rt_pt_params *prm,
         b->h_off[n + 1 + i] = (uint32_t)tiles;
         pix += (uint64_t)c.width * c.height;
         tiles += (uint64_t)t.tiles_x * ((c.height + 7) / 8);
-        if (pix >= (1ull << 31)) {
-            set_error("batch frames hold more than 2^31 pixels");
+        if (pix >= (1ull << 31) || tiles * 64u >= (1ull << 32)) {
+            set_error("batch frames hold more than 2^31 pixels (or 2^26 tiles of 8x8)");
             return RT_ERR_INVALID;
         }
     }

@@ -42,6 +42,7 @@ struct PtDev {
     uint32_t *fctr;       // trace-kernel band counters: [kMaxDepth][extend, shadow][8], 32-word stride,
                           // cleared at the start of each batch
     uint32_t share_trace; // trace-queue scheduling policy (BandSched share)
+    uint32_t refill;      // trace kernels: idle lanes of a wave that trigger a refill (trace_pool)
     // batched multi-scene mode (nscenes > 0): scene table, pixel and primary-tile
     // offsets of each scene (nscenes + 1 entries), per-slot scene index, samples
     // per batch; npix is then the pixel total

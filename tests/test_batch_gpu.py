@@ -29,10 +29,12 @@ def test_batch_equals_single_renders(members, batch_paths):
     from rendertoy_amd import capi
     print("case batch_paths", batch_paths, flush=True)
     b = capi.Batch(members)
-    outs = b.render(spp=4, max_depth=5, seed=11, batch_paths=batch_paths)
-    for s, o in zip(members, outs):
-        ref = s.render_pt(spp=4, max_depth=5, seed=11)
-        assert np.array_equal(bits(o), bits(ref)), s.sc.name
+    refs = [s.render_pt(spp=4, max_depth=5, seed=11) for s in members]
+    # three renders: the tuner's trials, every trace-queue policy once
+    for trial in range(3):
+        outs = b.render(spp=4, max_depth=5, seed=11, batch_paths=batch_paths)
+        for s, o, ref in zip(members, outs, refs):
+            assert np.array_equal(bits(o), bits(ref)), (s.sc.name, trial)
     st = b.stats()
     assert st["samples"] == sum(len(o) for o in outs) * 4
     assert st["rays"] > st["primary_rays"] > 0

@@ -119,13 +119,16 @@ hipError_t launch_rays(const DevScene &sc, const float *src, const float *dir, c
     return hipGetLastError();
 }
 
-__global__ void reset_u32_kernel(uint32_t *p, uint32_t n) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Zeroes p[0, n) and, if q is not null, q[0, m): one launch for both.
+__global__ void reset_u32_kernel(uint32_t *p, uint32_t n, uint32_t *q, uint32_t m) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+    if (q && i < m) q[i] = 0u;
 }
 
-hipError_t launch_reset(uint32_t *p, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(reset_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+hipError_t launch_reset(uint32_t *p, uint32_t n, hipStream_t s, uint32_t *q, uint32_t m) {
+    const uint32_t k = n > m ? n : m;
+    hipLaunchKernelGGL(reset_u32_kernel, dim3((k + 255) / 256), dim3(256), 0, s, p, n, q, m);
     return hipGetLastError();
 }
 

@@ -553,8 +553,7 @@ int rt_render_aov(rt_scene *s, const rt_camera *cam, uint32_t passes, const rt_a
         const uint32_t grid = persistent_grid(s, aov_blocks_per_cu(count));
         ensure_ovf(s, grid * 256);
         const int64_t ovf_stride = (int64_t)grid * 256;
-        RT_HIP(launch_reset(s->d_counters, 8 * 32, s->stream));
-        RT_HIP(launch_reset((uint32_t *)s->d_stat, 16, s->stream));
+        RT_HIP(launch_reset(s->d_counters, 8 * 32, s->stream, (uint32_t *)s->d_stat, 16));
         RT_HIP(hipEventRecord(s->ev[0], s->stream));
         RT_HIP(launch_aov(s->ds, *cam, s->aov, passes, s->d_counters, grid, s->d_ovf_ref, s->d_ovf_t, ovf_stride,
                           s->d_stat, count, s->stream));

@@ -18,7 +18,7 @@ struct AovDev {
     float *mask;
 };
 
-hipError_t launch_reset(uint32_t *p, uint32_t n, hipStream_t s);
+hipError_t launch_reset(uint32_t *p, uint32_t n, hipStream_t s, uint32_t *q = nullptr, uint32_t m = 0);
 int aov_blocks_per_cu(bool count);
 hipError_t launch_aov(const DevScene &sc, const rt_camera &cam, const AovDev &out, uint32_t passes,
                       uint32_t *counters, uint32_t grid, uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride,

@@ -387,6 +387,14 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    # VALU busy fraction and lane utilisation of the same kernel (rocprofv3 --pmc SQ_* counters)
+    valu = None
+    vf = os.path.join(ROOT, "profiles", "aov_c1_valu.json")
+    if os.path.exists(vf):
+        with open(vf) as f:
+            v = json.load(f)
+        valu = {"busy": v["valu_busy"], "lane_utilization": v["valu_lane_utilization"],
+                "instructions_per_launch": v["valu_instructions"]}
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -399,7 +407,11 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "aov_kernel", "kernel_ms": round(k_ms, 5),
                      "per_ray": {"node_fetches": round(n_node, 3), "tri_tests": round(n_tri, 3),
-                                 "bytes": round(bytes_launch / npix, 1)}},
+                                 "bytes": round(bytes_launch / npix, 1)},
+                     "valu": valu,
+                     "note": "algorithmic bytes count every BVH node and triangle fetch; they are served "
+                             "from L2/MALL (traffic = measured HBM bytes per launch), so frac can exceed 1; "
+                             "the loop is bound by VALU issue and lane divergence (valu, profiles/)"},
     }
     log("C1: %.1f Msamples/s, kernel %.4f ms, %.1f GB/s algorithmic" % (value, k_ms, achieved))
     if not args.no_pt:

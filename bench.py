@@ -100,6 +100,24 @@ def cpu_baseline_c0(sc, spp):
                 sc.width, sc.height, spp)}
 
 
+def cpu_baseline_c2(sc, W=24, H=14, spp=16):
+    """RenderToy's PathTracingRenderer on a downscaled C2 frame (same camera and
+    scene; OpenMP runs over samples, so spp = 16 keeps 16 threads busy).  The
+    reference's depth is fixed at 5, the device renders C2 at depth 8."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import py_oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    try:
+        ref = py_oracle.RefLib()
+    except (FileNotFoundError, OSError):
+        return None
+    w = ref.world(sc)
+    _, ms = w.render("pt", W, H, spp=spp)
+    return {"value": W * H * spp / ms / 1e3, "unit": "Msamples/s", "cores": threads, "kind": "reference",
+            "sample": "%dx%d frame of the same scene and camera at %d spp, depth 5 (RenderToy's fixed depth; "
+                      "the device runs depth 8), PathTracingRenderer::Render, %.1f s" % (W, H, spp, ms / 1e3)}
+
+
 def bench_c3(n_frames=2):
     """64 terrain scenes (10k-500k triangles, 128^2..1024^2, 32 spp, depth 6)
     in one batched wavefront, against the same frames rendered one scene at a time."""
@@ -450,6 +468,11 @@ def main():
                             "msamples_per_s": round(sc2.width * sc2.height * sc2.spp * n2 / dt / 1e6, 2),
                             "mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / n2, 2)}
         del s2
+        if not args.no_cpu:
+            cb = cpu_baseline_c2(sc2)
+            if cb:
+                pt["c2_terrain"]["cpu_baseline"] = cb
+                pt["c2_terrain"]["speedup_vs_cpu"] = round(pt["c2_terrain"]["msamples_per_s"] / cb["value"], 1)
         pt["c3_batch"] = bench_c3()
         out["path_tracer"] = pt
         out["post_process"] = bench_post(not args.no_cpu)

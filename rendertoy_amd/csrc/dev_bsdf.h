@@ -12,6 +12,15 @@ namespace rtamd {
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kE = 2.71828182845904523536f;   // M_Ef32
 
+// Beer-Lambert factor pow(E, -a * len) of one channel.  For a = +-0 and a
+// finite len the exponent is +-0 and pow(x, +-0) = 1 exactly, so the call is
+// skipped; the result is bit-identical either way.
+__device__ __forceinline__ float beer(float a, float len) {
+    return (a == 0.0f && len < __builtin_inff()) ? 1.0f : powf(kE, -a * len);
+}
+// log of one extinction channel; log(1) = +0 exactly in IEEE arithmetic.
+__device__ __forceinline__ float log_ext(float x) { return x == 1.0f ? 0.0f : logf(x); }
+
 __device__ __forceinline__ float lerpf_(float x, float y, float a) { return x * (1.0f - a) + y * a; }
 __device__ __forceinline__ V3 lerpv(V3 x, V3 y, V3 a) { return add(mulv(x, sub(v3(1.0f), a)), mulv(y, a)); }
 __device__ __forceinline__ float luma(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }

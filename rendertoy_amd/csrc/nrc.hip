@@ -623,7 +623,7 @@ __global__ __launch_bounds__(256) void nrc_gen_kernel(DevScene sc, NrcGen gp, fl
                 const float pdf_e = emission_pdf(pos, o, neg(d), N, sc.geo[h.id].w);
                 ev[dep] = dep == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
                 const float len = length(sub(pos, o));
-                const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
+                const V3 br = {beer(absn.x, len), beer(absn.y, len), beer(absn.z, len)};
                 float *f = feat[dep];
                 f[0] = pos.x; f[1] = pos.y; f[2] = pos.z;
                 f[3] = -d.x; f[4] = -d.y; f[5] = -d.z;
@@ -667,7 +667,7 @@ __global__ __launch_bounds__(256) void nrc_gen_kernel(DevScene sc, NrcGen gp, fl
                 V3 nd, color;
                 float bpdf;
                 if (!BsdfSample(m, eta, ff, neg(d), u01(ra.w), u01(rb.x), nd, color, bpdf)) break;
-                absn = divs(neg(V3{logf(m.extinction.x), logf(m.extinction.y), logf(m.extinction.z)}), m.at_distance);
+                absn = divs(neg(V3{log_ext(m.extinction.x), log_ext(m.extinction.y), log_ext(m.extinction.z)}), m.at_distance);
                 if (!(bpdf > 0.0f)) break;
                 wv[dep] = divs(mulv(br, color), bpdf);
                 o = pos;

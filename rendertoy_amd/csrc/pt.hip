@@ -186,7 +186,7 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
             p.thr[slot] = make_float4(T.x, T.y, T.z, 0.0f);
         }
         const float len = length(sub(pos, o));
-        const V3 br = {powf(kE, -absn.x * len), powf(kE, -absn.y * len), powf(kE, -absn.z * len)};
+        const V3 br = {beer(absn.x, len), beer(absn.y, len), beer(absn.z, len)};
         // DirectLight (src/renderer.cpp:335-384)
         if (sc.num_emit > 0 && !term) {
             const EmitterRec &E = sc.emit[pick(ra.x, sc.num_emit)];
@@ -226,7 +226,7 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
             const uint4 rb = philox(make_uint4(pixel, sample, depth, 1u), p.key0, p.key1);
             V3 color;
             if (BsdfSample(m, eta, ff, neg(d), u01(ra.w), u01(rb.x), nd, color, bpdf)) {
-                absn = divs(neg(V3{logf(m.extinction.x), logf(m.extinction.y), logf(m.extinction.z)}),
+                absn = divs(neg(V3{log_ext(m.extinction.x), log_ext(m.extinction.y), log_ext(m.extinction.z)}),
                             m.at_distance);
                 if (bpdf > 0.0f) {
                     T = mulv(T, divs(mulv(br, color), bpdf));

@@ -5,6 +5,8 @@
 // best-first search.  Here: a binned-SAH binary tree (32 bins per axis,
 // leaves of at most kMaxLeaf triangles) collapsed into 4-wide nodes of one
 // 128-byte line each, nodes laid out depth-first so a subtree is contiguous.
+// The build runs on the host's threads (subtrees as tasks, the binning of
+// large nodes in slices); the tree does not depend on the thread count.
 //
 // Closest-hit results do not depend on the tree: traversal keeps the minimum
 // (t, triangle index) over every triangle whose box the ray reaches, which is
@@ -14,11 +16,13 @@
 #include "bvh_build.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <future>
 #include <limits>
+#include <memory>
 #include <stdexcept>
 
 namespace rtamd {
@@ -44,108 +48,188 @@ struct Aabb {
 
 struct BNode {
     Aabb box;
-    int left = -1, right = -1;
-    uint32_t start = 0, count = 0;
+    int left, right;
+    uint32_t start, count;
     bool leaf() const { return left < 0; }
 };
 
 constexpr int kBins = 32;
 
+// SAH bins of one axis over a range of triangles.
+struct Bins {
+    Aabb box[3][kBins];
+    uint32_t cnt[3][kBins];
+    void reset() {
+        for (int ax = 0; ax < 3; ++ax)
+            for (int b = 0; b < kBins; ++b) {
+                box[ax][b].reset();
+                cnt[ax][b] = 0;
+            }
+    }
+    void merge(const Bins &o) {
+        for (int ax = 0; ax < 3; ++ax)
+            for (int b = 0; b < kBins; ++b) {
+                box[ax][b].grow(o.box[ax][b]);
+                cnt[ax][b] += o.cnt[ax][b];
+            }
+    }
+};
+
+// Nodes at least this large bin their triangles on several threads; subtrees
+// at least kTaskTris large are built on their own thread.  Bins merge by
+// min/max and integer sums, so the tree does not depend on the split.
+constexpr uint32_t kParallelBinTris = 1u << 18;
+constexpr uint32_t kBinChunks = 16;
+constexpr uint32_t kTaskTris = 1u << 14;
+
+// A triangle as the builder sees it: its box, its box centre, its index.
+// The build partitions these records themselves, so every pass over a node's
+// range reads memory in order.
+struct Prim {
+    Aabb box;
+    float c[3];
+    uint32_t id;
+};
+
 struct Builder {
-    const std::vector<Aabb> &boxes;
-    std::vector<float> cen;          // 3 per triangle
-    std::vector<uint32_t> &idx;
-    std::vector<BNode> nodes;
-    std::mutex mu;
+    Prim *prim;
+    std::unique_ptr<BNode[]> nodes;  // the worst case (2n - 1)
+    std::atomic<int> next{0};
 
-    Builder(const std::vector<Aabb> &b, std::vector<uint32_t> &order) : boxes(b), idx(order) {}
+    explicit Builder(Prim *p) : prim(p) {}
 
-    int alloc() {
-        std::lock_guard<std::mutex> g(mu);
-        nodes.emplace_back();
-        return (int)nodes.size() - 1;
+    // Node indices depend on thread timing; the tree's shape does not, and the
+    // 4-wide layout is written by a depth-first walk of that shape.
+    int alloc() { return next.fetch_add(1, std::memory_order_relaxed); }
+
+    // Box of the triangles and of their centres over prim[begin, end).
+    void bounds(uint32_t begin, uint32_t end, Aabb &box, Aabb &cbox) const {
+        Aabb bx, cb;
+        bx.reset();
+        cb.reset();
+        for (uint32_t i = begin; i < end; ++i) {
+            bx.grow(prim[i].box);
+            cb.grow_pt(prim[i].c);
+        }
+        box = bx;
+        cbox = cb;
     }
 
-    // Builds the subtree over idx[begin, end); returns its node index.
-    int build(uint32_t begin, uint32_t end, int depth) {
-        int me = alloc();
-        Aabb box, cbox;
-        box.reset();
-        cbox.reset();
+    void bin_range(uint32_t begin, uint32_t end, const Aabb &cbox, const float scale[3], Bins &out) const {
+        out.reset();
         for (uint32_t i = begin; i < end; ++i) {
-            box.grow(boxes[idx[i]]);
-            cbox.grow_pt(&cen[3 * idx[i]]);
+            const Prim &t = prim[i];
+            for (int ax = 0; ax < 3; ++ax) {
+                if (!(scale[ax] > 0.0f)) continue;
+                const int b = std::min(kBins - 1, (int)((t.c[ax] - cbox.lo[ax]) * scale[ax]));
+                out.box[ax][b].grow(t.box);
+                out.cnt[ax][b]++;
+            }
         }
-        uint32_t n = end - begin;
+    }
+
+    // Runs f(c, b0, b1) over kBinChunks slices of [begin, end) on their own threads.
+    template <class F>
+    void in_slices(uint32_t begin, uint32_t end, F &&f) {
+        const uint32_t n = end - begin;
+        std::vector<std::future<void>> fut;
+        for (uint32_t c = 0; c < kBinChunks; ++c) {
+            const uint32_t b0 = begin + (uint32_t)((uint64_t)n * c / kBinChunks);
+            const uint32_t b1 = begin + (uint32_t)((uint64_t)n * (c + 1) / kBinChunks);
+            fut.push_back(std::async(std::launch::async, [&f, c, b0, b1]() { f(c, b0, b1); }));
+        }
+        for (auto &x : fut) x.get();
+    }
+
+    // Builds the subtree over prim[begin, end); returns its node index.
+    int build(uint32_t begin, uint32_t end, int depth) {
+        const int me = alloc();
+        const uint32_t n = end - begin;
+        const bool wide = n >= kParallelBinTris;
+        Aabb box, cbox;
+        if (wide) {
+            Aabb pb[kBinChunks], pc[kBinChunks];
+            in_slices(begin, end, [&](uint32_t c, uint32_t b0, uint32_t b1) { bounds(b0, b1, pb[c], pc[c]); });
+            box = pb[0];
+            cbox = pc[0];
+            for (uint32_t c = 1; c < kBinChunks; ++c) {
+                box.grow(pb[c]);
+                cbox.grow(pc[c]);
+            }
+        } else {
+            bounds(begin, end, box, cbox);
+        }
         auto make_leaf = [&]() {
-            std::lock_guard<std::mutex> g(mu);
-            nodes[me].box = box;
-            nodes[me].start = begin;
-            nodes[me].count = n;
+            BNode &nd = nodes[me];
+            nd.box = box;
+            nd.left = nd.right = -1;
+            nd.start = begin;
+            nd.count = n;
             return me;
         };
-        if (n <= 2) {
-            if (n == 1) return make_leaf();
-        }
+        if (n == 1) return make_leaf();
         // binned SAH over the centroid box
+        float scale[3];
+        for (int ax = 0; ax < 3; ++ax) {
+            const float ext = cbox.hi[ax] - cbox.lo[ax];
+            scale[ax] = ext > 0.0f ? kBins / ext : 0.0f;
+        }
+        Bins bins;
+        if (wide) {
+            std::vector<Bins> part(kBinChunks);
+            in_slices(begin, end, [&](uint32_t c, uint32_t b0, uint32_t b1) { bin_range(b0, b1, cbox, scale, part[c]); });
+            bins = part[0];
+            for (uint32_t c = 1; c < kBinChunks; ++c) bins.merge(part[c]);
+        } else {
+            bin_range(begin, end, cbox, scale, bins);
+        }
         int best_axis = -1, best_bin = -1;
         float best_cost = std::numeric_limits<float>::infinity();
         for (int ax = 0; ax < 3; ++ax) {
-            float ext = cbox.hi[ax] - cbox.lo[ax];
-            if (!(ext > 0.0f)) continue;
-            float scale = kBins / ext;
-            Aabb bb[kBins];
-            uint32_t bc[kBins] = {};
-            for (int b = 0; b < kBins; ++b) bb[b].reset();
-            for (uint32_t i = begin; i < end; ++i) {
-                int b = std::min(kBins - 1, (int)((cen[3 * idx[i] + ax] - cbox.lo[ax]) * scale));
-                bb[b].grow(boxes[idx[i]]);
-                bc[b]++;
-            }
+            if (!(scale[ax] > 0.0f)) continue;
             float ra[kBins];
             uint32_t rc[kBins];
             Aabb acc;
             acc.reset();
             uint32_t cnt = 0;
             for (int b = kBins - 1; b > 0; --b) {
-                acc.grow(bb[b]);
-                cnt += bc[b];
+                acc.grow(bins.box[ax][b]);
+                cnt += bins.cnt[ax][b];
                 ra[b] = acc.area();
                 rc[b] = cnt;
             }
             acc.reset();
             cnt = 0;
             for (int b = 0; b < kBins - 1; ++b) {
-                acc.grow(bb[b]);
-                cnt += bc[b];
+                acc.grow(bins.box[ax][b]);
+                cnt += bins.cnt[ax][b];
                 if (cnt == 0 || rc[b + 1] == 0) continue;
-                float cost = acc.area() * cnt + ra[b + 1] * rc[b + 1];
+                const float cost = acc.area() * cnt + ra[b + 1] * rc[b + 1];
                 if (cost < best_cost) { best_cost = cost; best_axis = ax; best_bin = b; }
             }
         }
-        float pa = box.area();
-        float leaf_cost = (float)n;
-        float split_cost = pa > 0.0f ? 1.0f + best_cost / pa : std::numeric_limits<float>::infinity();
+        const float pa = box.area();
+        const float leaf_cost = (float)n;
+        const float split_cost = pa > 0.0f ? 1.0f + best_cost / pa : std::numeric_limits<float>::infinity();
         if (best_axis < 0 || (n <= 4 && leaf_cost <= split_cost)) {
             if (n <= kMaxLeaf) return make_leaf();
         }
         uint32_t mid;
         if (best_axis >= 0) {
-            float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
-            float scale = kBins / ext;
-            float lo = cbox.lo[best_axis];
-            auto it = std::partition(idx.begin() + begin, idx.begin() + end, [&](uint32_t t) {
-                int b = std::min(kBins - 1, (int)((cen[3 * t + best_axis] - lo) * scale));
+            const float sc = scale[best_axis];
+            const float lo = cbox.lo[best_axis];
+            auto it = std::partition(prim + begin, prim + end, [&](const Prim &t) {
+                const int b = std::min(kBins - 1, (int)((t.c[best_axis] - lo) * sc));
                 return b <= best_bin;
             });
-            mid = (uint32_t)(it - idx.begin());
+            mid = (uint32_t)(it - prim);
             if (mid == begin || mid == end) mid = begin + n / 2;
         } else {
-            // all centroids coincide: split by count along the longest axis of the box
+            // all centroids coincide: split by count
             mid = begin + n / 2;
         }
         int l, r;
-        if (n > 65536 && depth < 6) {
+        if (n >= kTaskTris && depth < 12) {
             auto fut = std::async(std::launch::async, [&]() { return build(begin, mid, depth + 1); });
             r = build(mid, end, depth + 1);
             l = fut.get();
@@ -153,10 +237,12 @@ struct Builder {
             l = build(begin, mid, depth + 1);
             r = build(mid, end, depth + 1);
         }
-        std::lock_guard<std::mutex> g(mu);
-        nodes[me].box = box;
-        nodes[me].left = l;
-        nodes[me].right = r;
+        BNode &nd = nodes[me];
+        nd.box = box;
+        nd.left = l;
+        nd.right = r;
+        nd.start = begin;
+        nd.count = n;   // triangles under an inner node: sizes the collapse's tasks
         return me;
     }
 };
@@ -173,37 +259,72 @@ inline float up(double x) {
     return f;
 }
 
+// Collapse of the binary tree into 4-wide nodes, laid out depth first.  A
+// first pass counts the 4-wide nodes under each collapsed root, so the second
+// can write subtrees at known offsets on several threads; the layout is the
+// one a serial depth-first walk produces.
 struct Collapser {
-    const std::vector<BNode> &b2;
+    const BNode *b2;
+    std::vector<uint32_t> size;   // 4-wide nodes under binary node i, for the roots of collapsed nodes
     std::vector<Node4> &out;
-    uint32_t max_depth = 0;
+    static constexpr uint32_t kTaskTris = 1u << 15;
 
-    // Returns (node index, stack bound of its subtree).
-    std::pair<uint32_t, uint32_t> collapse(int root, uint32_t depth) {
-        int kids[4];
+    // The up to 4 binary nodes that become the children of root's 4-wide node:
+    // repeatedly open the inner child with the largest box.
+    int kids_of(int root, int kids[4]) const {
         int nk = 0;
         if (b2[root].leaf()) {
             kids[nk++] = root;
-        } else {
-            kids[nk++] = b2[root].left;
-            kids[nk++] = b2[root].right;
-            while (nk < 4) {
-                int best = -1;
-                float ba = -1.0f;
-                for (int i = 0; i < nk; ++i)
-                    if (!b2[kids[i]].leaf() && b2[kids[i]].box.area() > ba) { ba = b2[kids[i]].box.area(); best = i; }
-                if (best < 0) break;
-                int c = kids[best];
-                kids[best] = b2[c].left;
-                kids[nk++] = b2[c].right;
-            }
+            return nk;
         }
-        uint32_t me = (uint32_t)out.size();
-        out.emplace_back();
-        max_depth = std::max(max_depth, depth + 1);
+        kids[nk++] = b2[root].left;
+        kids[nk++] = b2[root].right;
+        while (nk < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int i = 0; i < nk; ++i)
+                if (!b2[kids[i]].leaf() && b2[kids[i]].box.area() > ba) { ba = b2[kids[i]].box.area(); best = i; }
+            if (best < 0) break;
+            const int c = kids[best];
+            kids[best] = b2[c].left;
+            kids[nk++] = b2[c].right;
+        }
+        return nk;
+    }
+
+    // Pass 1: 4-wide nodes under root, recorded in size[root].
+    uint32_t count(int root, int depth) {
+        int kids[4];
+        const int nk = kids_of(root, kids);
+        uint32_t total = 1;
+        std::future<uint32_t> fut[4];
+        for (int j = 0; j < nk; ++j) {
+            if (b2[kids[j]].leaf()) continue;
+            if (b2[kids[j]].count >= kTaskTris && depth < 8)
+                fut[j] = std::async(std::launch::async, [this, k = kids[j], depth]() { return count(k, depth + 1); });
+            else
+                total += count(kids[j], depth + 1);
+        }
+        for (int j = 0; j < nk; ++j)
+            if (fut[j].valid()) total += fut[j].get();
+        size[root] = total;
+        return total;
+    }
+
+    struct Res {
+        uint32_t depth, stack;   // 4-wide levels and stack bound of the subtree
+    };
+
+    // Pass 2: writes root's 4-wide node at out[at] and its subtrees after it.
+    Res fill(int root, uint32_t at, int depth) {
+        int kids[4];
+        const int nk = kids_of(root, kids);
         Node4 nd;
         std::memset(&nd, 0, sizeof(nd));
-        uint32_t sub = 0;
+        uint32_t next = at + 1;
+        uint32_t sub_stack = 0, sub_depth = 0;
+        std::future<Res> fut[4];
+        Res res[4] = {};
         for (int j = 0; j < 4; ++j) {
             if (j >= nk) {
                 nd.child[j] = kEmptyRef;
@@ -226,13 +347,22 @@ struct Collapser {
             if (c.leaf()) {
                 nd.child[j] = make_leaf(c.start, c.count);
             } else {
-                auto r = collapse(kids[j], depth + 1);
-                nd.child[j] = r.first;
-                sub = std::max(sub, r.second);
+                nd.child[j] = next;
+                if (c.count >= kTaskTris && depth < 8)
+                    fut[j] = std::async(std::launch::async,
+                                        [this, k = kids[j], next, depth]() { return fill(k, next, depth + 1); });
+                else
+                    res[j] = fill(kids[j], next, depth + 1);
+                next += size[kids[j]];
             }
         }
-        out[me] = nd;
-        return {me, (uint32_t)(nk - 1) + sub};
+        for (int j = 0; j < nk; ++j) {
+            if (fut[j].valid()) res[j] = fut[j].get();
+            sub_stack = std::max(sub_stack, res[j].stack);
+            sub_depth = std::max(sub_depth, res[j].depth);
+        }
+        out[at] = nd;
+        return {sub_depth + 1, (uint32_t)(nk - 1) + sub_stack};
     }
 };
 
@@ -255,37 +385,56 @@ Bvh4 build_bvh4(uint32_t n, const float *v0e1e2) {
         return res;
     }
     if (n >= (1u << 28)) throw std::runtime_error("scene exceeds 2^28 triangles");
-    std::vector<Aabb> boxes(n);
-    res.order.resize(n);
-    std::vector<uint32_t> &order = res.order;
-    for (uint32_t i = 0; i < n; ++i) {
-        const float *p = v0e1e2 + 9 * (size_t)i;
-        double a[3][3];
-        for (int k = 0; k < 3; ++k) {
-            a[0][k] = p[k];
-            a[1][k] = (double)p[k] + (double)p[3 + k];
-            a[2][k] = (double)p[k] + (double)p[6 + k];
+    std::unique_ptr<Prim[]> prim_store(new Prim[n]);   // written by prep, in parallel slices
+    Prim *prim = prim_store.get();
+    // triangle boxes (rounded outward from double) and centres, in parallel slices
+    auto prep = [&](uint32_t i0, uint32_t i1) {
+        for (uint32_t i = i0; i < i1; ++i) {
+            const float *p = v0e1e2 + 9 * (size_t)i;
+            double a[3][3];
+            for (int k = 0; k < 3; ++k) {
+                a[0][k] = p[k];
+                a[1][k] = (double)p[k] + (double)p[3 + k];
+                a[2][k] = (double)p[k] + (double)p[6 + k];
+            }
+            Prim &t = prim[i];
+            for (int k = 0; k < 3; ++k) {
+                double lo = std::min(a[0][k], std::min(a[1][k], a[2][k]));
+                double hi = std::max(a[0][k], std::max(a[1][k], a[2][k]));
+                t.box.lo[k] = down(lo);
+                t.box.hi[k] = up(hi);
+                t.c[k] = 0.5f * (t.box.lo[k] + t.box.hi[k]);
+            }
+            t.id = i;
         }
-        for (int k = 0; k < 3; ++k) {
-            double lo = std::min(a[0][k], std::min(a[1][k], a[2][k]));
-            double hi = std::max(a[0][k], std::max(a[1][k], a[2][k]));
-            boxes[i].lo[k] = down(lo);
-            boxes[i].hi[k] = up(hi);
-        }
-        order[i] = i;
+    };
+    Builder b(prim);
+    if (n >= kParallelBinTris)
+        b.in_slices(0, n, [&](uint32_t, uint32_t i0, uint32_t i1) { prep(i0, i1); });
+    else
+        prep(0, n);
+    b.nodes.reset(new BNode[2 * (size_t)n]);
+    {
+        // zeroed in parallel slices: first touches of fresh pages on one thread cost more than the build's top levels
+        char *base = reinterpret_cast<char *>(b.nodes.get());
+        const size_t tot = 2 * (size_t)n * sizeof(BNode);
+        if (n >= kParallelBinTris)
+            b.in_slices(0, kBinChunks, [&](uint32_t c, uint32_t, uint32_t) {
+                std::memset(base + tot * c / kBinChunks, 0, tot * (c + 1) / kBinChunks - tot * c / kBinChunks);
+            });
+        else
+            std::memset(base, 0, tot);
     }
-    Builder b(boxes, order);
-    b.cen.resize(3 * (size_t)n);
-    for (uint32_t i = 0; i < n; ++i)
-        for (int k = 0; k < 3; ++k) b.cen[3 * i + k] = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
-    b.nodes.reserve(2 * (size_t)n);
     int root = b.build(0, n, 0);
     if (root != 0) throw std::runtime_error("bvh root is not node 0");
-    Collapser c{b.nodes, res.nodes};
-    res.nodes.reserve(b.nodes.size() / 2 + 1);
-    auto r = c.collapse(0, 0);
-    res.depth = c.max_depth;
-    res.max_stack = r.second;
+    res.order.resize(n);
+    for (uint32_t i = 0; i < n; ++i) res.order[i] = prim[i].id;
+    Collapser c{b.nodes.get(), std::vector<uint32_t>((size_t)b.next.load()), res.nodes};
+    const uint32_t total = c.count(0, 0);
+    res.nodes.resize(total);
+    const Collapser::Res r = c.fill(0, 0, 0);
+    res.depth = r.depth;
+    res.max_stack = r.stack;
     res.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return res;
 }

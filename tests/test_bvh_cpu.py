@@ -1,0 +1,62 @@
+"""The host BVH4 builder (rendertoy_amd/csrc/bvh_build.cpp) on the CPU.
+
+tests/cpp/bvh_check.cpp builds a triangle set twice and checks determinism
+(the build is threaded), that the leaves tile the triangle list exactly once,
+that every box contains what lies below it (leaf triangles by their outward
+rounded boxes), and the stack bound the traversal kernels size their
+overflow region with.  Sets: random, all triangles on one centroid, tiny
+(1, 2, 3, 9 triangles) and one large enough for the parallel binning paths.
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("bvh") / "bvh_check")
+    subprocess.run(["g++", "-std=c++20", "-O2", "-pthread", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "bvh_check.cpp"),
+                    os.path.join(ROOT, "rendertoy_amd", "csrc", "bvh_build.cpp"), "-o", exe], check=True)
+
+    def run(tris, tmp):
+        tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        path = os.path.join(str(tmp), "tris.bin")
+        with open(path, "wb") as f:
+            f.write(struct.pack("<I", len(tris)))
+            f.write(tris.tobytes())
+        r = subprocess.run([exe, path], capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout + r.stderr
+        return r.stdout.split()
+    return run
+
+
+def random_tris(n, seed, spread=10.0, size=0.05):
+    rng = np.random.default_rng(seed)
+    v0 = rng.uniform(-spread, spread, (n, 3))
+    e = rng.normal(0.0, size, (n, 6))
+    return np.concatenate([v0, e], axis=1)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 9, 1000])
+def test_bvh_random(checker, tmp_path, n):
+    out = checker(random_tris(n, n), tmp_path)
+    assert out[0] == "OK"
+
+
+def test_bvh_coincident_centroids(checker, tmp_path):
+    """5,000 copies of one triangle: no SAH split exists, counts split evenly."""
+    t = np.tile(np.array([[0.5, -1.0, 2.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0]]), (5000, 1))
+    assert checker(t, tmp_path)[0] == "OK"
+
+
+def test_bvh_large_parallel_paths(checker, tmp_path):
+    """300k triangles: nodes of 2^18 and more bin and bound on several threads."""
+    t = random_tris(300_000, 7, spread=50.0, size=0.2)
+    t[::3, :3] *= 1e-3                      # a dense cluster near the origin among the spread
+    assert checker(t, tmp_path)[0] == "OK"

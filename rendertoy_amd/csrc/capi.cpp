@@ -325,17 +325,18 @@ int rt_scene_create(const rt_scene_desc *desc, rt_scene **out) {
 
 // ---------------------------------------------------------------- binary scene cache
 //
-// File (little-endian): "RTAMDSCN", u32 version 1, then u32 num_tris,
+// File (little-endian): "RTAMDSCN", u32 version 2, then u32 num_tris,
 // num_nodes, num_materials, num_emitters, has_vnorm, bvh_depth, max_stack,
 // f32 sky[3], ground[3], bbox[6]; then the device arrays as uploaded: BVH4
 // nodes (128 B each), leaf triangles (3 float4 per triangle), geo (float4),
 // info (u32), vertex normals (9 floats per triangle, if any), materials
-// (rt_material), emitters (EmitterRec); then an FNV-1a 64 checksum of all
-// preceding bytes.  Loading uploads them as they are: no triangle caches, no BVH build.
+// (rt_material), emitters (EmitterRec); then a u64 checksum of all preceding
+// bytes (scene_sum).  Loading uploads them as they are: no triangle caches,
+// no BVH build.
 
 namespace {
 constexpr char kSceneMagic[8] = {'R', 'T', 'A', 'M', 'D', 'S', 'C', 'N'};
-constexpr uint32_t kSceneVersion = 1;
+constexpr uint32_t kSceneVersion = 2;
 
 struct SceneHeader {
     char magic[8];
@@ -343,13 +344,38 @@ struct SceneHeader {
     float sky[3], ground[3], bbox[6];
 };
 
-uint64_t fnv1a(uint64_t h, const void *p, size_t n) {
+constexpr uint64_t kFnvBasis = 1469598103934665603ull, kFnvPrime = 1099511628211ull;
+
+// Checksum of a scene file: FNV-1a 64 run as 8 interleaved lanes over
+// little-endian 64-bit words (word i feeds lane i % 8, a partial last word is
+// zero-padded), then the 8 lane values, lane 0 first, and the byte count
+// folded into one more FNV-1a 64 word by word.  Independent lanes keep 8
+// multiplies in flight; byte-wise FNV-1a ran at about 1 GB/s.
+uint64_t scene_sum(const void *p, size_t n) {
     const uint8_t *b = static_cast<const uint8_t *>(p);
-    for (size_t i = 0; i < n; ++i) {
-        h ^= b[i];
-        h *= 1099511628211ull;
+    uint64_t lane[8];
+    for (auto &l : lane) l = kFnvBasis;
+    const size_t words = n / 8;
+    size_t i = 0;
+    for (; i + 8 <= words; i += 8)
+        for (int k = 0; k < 8; ++k) {
+            uint64_t w;
+            std::memcpy(&w, b + 8 * (i + k), 8);
+            lane[k] = (lane[k] ^ w) * kFnvPrime;
+        }
+    for (; i < words; ++i) {
+        uint64_t w;
+        std::memcpy(&w, b + 8 * i, 8);
+        lane[i % 8] = (lane[i % 8] ^ w) * kFnvPrime;
     }
-    return h;
+    if (n % 8) {
+        uint64_t w = 0;
+        std::memcpy(&w, b + 8 * words, n % 8);
+        lane[words % 8] = (lane[words % 8] ^ w) * kFnvPrime;
+    }
+    uint64_t h = kFnvBasis;
+    for (uint64_t l : lane) h = (h ^ l) * kFnvPrime;
+    return (h ^ (uint64_t)n) * kFnvPrime;
 }
 
 struct SceneArrays {
@@ -386,23 +412,21 @@ int rt_scene_save(const rt_scene *s, const char *path) {
         const SceneArrays a = array_sizes(h);
         const void *src[7] = {s->ds.nodes, s->ds.tris, s->ds.geo, s->ds.info, s->ds.vnorm, s->ds.mats, s->ds.emit};
         const size_t len[7] = {a.nodes, a.tris, a.geo, a.info, a.vnorm, a.mats, a.emit};
+        size_t payload = sizeof(h);
+        for (size_t l : len) payload += l;
+        std::vector<uint8_t> buf(payload + 8);
+        std::memcpy(buf.data(), &h, sizeof(h));
+        size_t at = sizeof(h);
+        for (int i = 0; i < 7; ++i) {
+            if (!len[i]) continue;
+            RT_HIP(hipMemcpy(buf.data() + at, src[i], len[i], hipMemcpyDeviceToHost));
+            at += len[i];
+        }
+        const uint64_t sum = scene_sum(buf.data(), payload);
+        std::memcpy(buf.data() + payload, &sum, 8);
         FILE *f = std::fopen(path, "wb");
         if (!f) throw InvalidArg{std::string("cannot open ") + path + " for writing"};
-        uint64_t sum = fnv1a(1469598103934665603ull, &h, sizeof(h));
-        bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
-        std::vector<uint8_t> buf;
-        for (int i = 0; i < 7 && ok; ++i) {
-            if (!len[i]) continue;
-            buf.resize(len[i]);
-            hipError_t e = hipMemcpy(buf.data(), src[i], len[i], hipMemcpyDeviceToHost);
-            if (e != hipSuccess) {
-                std::fclose(f);
-                throw HipError{e, "hipMemcpy"};
-            }
-            sum = fnv1a(sum, buf.data(), len[i]);
-            ok = std::fwrite(buf.data(), 1, len[i], f) == len[i];
-        }
-        ok = ok && std::fwrite(&sum, sizeof(sum), 1, f) == 1;
+        bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
         ok = (std::fclose(f) == 0) && ok;
         if (!ok) throw InvalidArg{std::string("write to ") + path + " failed"};
         return (int)RT_OK;
@@ -443,7 +467,7 @@ int rt_scene_load(const char *path, rt_scene **out) {
         if (data.size() != payload + 8) throw InvalidArg{"scene file size does not match its header"};
         uint64_t sum;
         std::memcpy(&sum, data.data() + payload, 8);
-        if (fnv1a(1469598103934665603ull, data.data(), payload) != sum) throw InvalidArg{"scene file checksum mismatch"};
+        if (scene_sum(data.data(), payload) != sum) throw InvalidArg{"scene file checksum mismatch"};
         check_device(s);
         RT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         for (auto &e : s->ev) RT_HIP(hipEventCreate(&e));

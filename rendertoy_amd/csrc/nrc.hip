@@ -454,7 +454,9 @@ __global__ __launch_bounds__(256) void nrc_train_bwd_kernel(NrcGrid g, const _Fl
 // Grid gradient: one thread per (sample, level) adds the two features'
 // input gradients, times the trilinear weights, to the level's 8 corners.  A
 // wave takes one level of 64 consecutive samples (spatially coherent when the
-// samples are), so its atomics land on nearby entries.
+// samples are), so its atomics land on nearby entries.  Dense (coarse) levels
+// add into shard lane % kGradShards instead of the gradient itself:
+// nrc_shard_reduce_kernel then folds the shards in.
 __global__ __launch_bounds__(256) void nrc_grid_scatter_kernel(NrcGrid g, const float *samples, const float *dgrid,
                                                                uint32_t n) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -467,7 +469,8 @@ __global__ __launch_bounds__(256) void nrc_grid_scatter_kernel(NrcGrid g, const 
     float p[3];
     unit_pos(g, samples + kNrcFloats * (size_t)q, p);
     const Corners c = level_cell(g, (int)l, p);
-    float *gg = g.grad + (size_t)l * (g.tmask + 1) * 2;
+    float *gg = g.dense[l] ? g.shard + (size_t)(t & (kGradShards - 1u)) * g.dense_floats + g.doff[l]
+                           : g.grad + (size_t)l * (g.tmask + 1) * 2;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         uint32_t idx;
@@ -476,6 +479,22 @@ __global__ __launch_bounds__(256) void nrc_grid_scatter_kernel(NrcGrid g, const 
         atomicAdd(gg + 2 * idx, wgt * gd.x);
         atomicAdd(gg + 2 * idx + 1, wgt * gd.y);
     }
+}
+
+// Dense-level gradient: sums the kGradShards copies of each entry into the
+// grid gradient and clears them for the next step.
+__global__ void nrc_shard_reduce_kernel(NrcGrid g) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.dense_floats) return;
+    int l = 0;
+    while (l + 1 < kNrcLevels && (!g.dense[l] || p >= g.doff[l] + 2u * g.side[l] * g.side[l] * g.side[l])) ++l;
+    float sum = 0.0f;
+    for (uint32_t k = 0; k < kGradShards; ++k) {
+        float *q = g.shard + (size_t)k * g.dense_floats + p;
+        sum += *q;
+        *q = 0.0f;
+    }
+    g.grad[(size_t)l * (g.tmask + 1) * 2 + (p - g.doff[l])] += sum;
 }
 
 // First pass of the MLP-gradient reduction: group k of kRedGroups sums the
@@ -723,6 +742,8 @@ hipError_t nrc_launch_train_step(const NrcGrid &g, NrcTrain &t, const float *sam
                        t.wpart, t.dgrid);
     const uint32_t threads = waves * 64u * (uint32_t)kNrcLevels;
     hipLaunchKernelGGL(nrc_grid_scatter_kernel, dim3((threads + 255) / 256), dim3(256), 0, s, g, samples, t.dgrid, n);
+    if (g.dense_floats)
+        hipLaunchKernelGGL(nrc_shard_reduce_kernel, dim3((g.dense_floats + 255) / 256), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 

@@ -141,6 +141,12 @@ int rt_cache_create(rt_scene *scene, const rt_cache_params *prm, rt_cache **out)
             g.binv[k] = 1.0f / (ext * 1.002f);
         }
         g.grid_params = (size_t)kNrcLevels * T * 2;
+        g.dense_floats = 0;
+        for (int l = 0; l < kNrcLevels; ++l) {
+            g.doff[l] = g.dense_floats;
+            if (g.dense[l]) g.dense_floats += 2u * g.side[l] * g.side[l] * g.side[l];
+        }
+        g.shard = g.dense_floats ? (float *)alloc(c, (size_t)kGradShards * g.dense_floats * sizeof(float)) : nullptr;
         NrcTrain &t = c->tr;
         t.w = (float *)alloc(c, kNrcParams * sizeof(float));
         t.m = (float *)alloc(c, kNrcParams * sizeof(float));
@@ -174,6 +180,7 @@ int rt_cache_create(rt_scene *scene, const rt_cache_params *prm, rt_cache **out)
         hipchk(nrc_launch_zero(t.grid_m, g.grid_params, st), "zero");
         hipchk(nrc_launch_zero(t.grid_v, g.grid_params, st), "zero");
         hipchk(nrc_launch_zero(g.grad, g.grid_params, st), "zero");
+        if (g.shard) hipchk(nrc_launch_zero(g.shard, (size_t)kGradShards * g.dense_floats, st), "zero");
         hipchk(nrc_launch_sync_weights(g, t, st), "sync");
         hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
         ensure_batch(c, std::max<uint32_t>(prm->train_batch, 64));

@@ -32,7 +32,18 @@ struct NrcGrid {
     float res[kNrcLevels];
     float bmin[3], binv[3];
     size_t grid_params;     // levels * T * 2
+    // gradient shards of the dense levels (see nrc_grid_scatter_kernel):
+    // kGradShards copies of [dense level l at doff[l]: side^3 x 2 floats]
+    float *shard;
+    uint32_t doff[kNrcLevels];
+    uint32_t dense_floats;  // floats per shard
 };
+
+// Copies of the dense levels' gradient that the scatter spreads its atomics
+// over (lane % kGradShards): coarse levels see dozens of atomics per entry per
+// step, which serialise on one address.
+constexpr uint32_t kGradShards = 16;
+static_assert((kGradShards & (kGradShards - 1)) == 0, "kGradShards is a power of two");
 
 struct NrcTrain {
     float *w, *m, *v;           // MLP fp32 master + Adam moments

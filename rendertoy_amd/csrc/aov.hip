@@ -15,15 +15,19 @@
 namespace rtamd {
 
 template <bool COUNT>
-RT_TRACE_KERNEL void aov_kernel(DevScene sc, rt_camera cam, AovDev out, uint32_t passes,
-                                                  uint32_t *counters, uint32_t tiles_x, uint32_t ntiles,
-                                                  uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride,
-                                                  unsigned long long *stat) {
+RT_TRACE_KERNEL void aov_kernel(DevScene sc, rt_camera cam, AovDev out, uint32_t passes, uint32_t *counters,
+                                uint32_t *next_counters, uint32_t tiles_x, uint32_t ntiles, uint32_t *ovf_ref,
+                                float *ovf_t, int64_t ovf_stride, unsigned long long *stat,
+                                unsigned long long *next_stat) {
     __shared__ uint32_t s_ref[kLdsStack * 256];
     __shared__ float s_t[kLdsStack * 256];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
+    if (blockIdx.x == 0) {   // the next launch's counters and statistics (this launch uses the other set)
+        next_counters[tid] = 0u;
+        if (tid < 8u) next_stat[tid] = 0ull;
+    }
     Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
     const float divW = 1.0f / float(cam.width);
     const float divH = 1.0f / float(cam.height);
@@ -132,26 +136,33 @@ hipError_t launch_reset(uint32_t *p, uint32_t n, hipStream_t s, uint32_t *q, uin
     return hipGetLastError();
 }
 
+// Occupancy of a kernel does not change within a process (one architecture): asked once.
 int aov_blocks_per_cu(bool count) {
-    int nb = 0;
-    if (count)
+    static const int nb_count = [] {
+        int nb = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aov_kernel<true>, 256, 0);
-    else
+        return nb > 0 ? nb : 1;
+    }();
+    static const int nb_plain = [] {
+        int nb = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, aov_kernel<false>, 256, 0);
-    return nb > 0 ? nb : 1;
+        return nb > 0 ? nb : 1;
+    }();
+    return count ? nb_count : nb_plain;
 }
 
 hipError_t launch_aov(const DevScene &sc, const rt_camera &cam, const AovDev &out, uint32_t passes,
-                      uint32_t *counters, uint32_t grid, uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride,
-                      unsigned long long *stat, bool count, hipStream_t s) {
+                      uint32_t *counters, uint32_t *next_counters, uint32_t grid, uint32_t *ovf_ref, float *ovf_t,
+                      int64_t ovf_stride, unsigned long long *stat, unsigned long long *next_stat, bool count,
+                      hipStream_t s) {
     const uint32_t tiles_x = (cam.width + 7) / 8, tiles_y = (cam.height + 7) / 8;
     const uint32_t ntiles = tiles_x * tiles_y;
     if (count)
-        hipLaunchKernelGGL(aov_kernel<true>, dim3(grid), dim3(256), 0, s, sc, cam, out, passes, counters, tiles_x,
-                           ntiles, ovf_ref, ovf_t, ovf_stride, stat);
+        hipLaunchKernelGGL(aov_kernel<true>, dim3(grid), dim3(256), 0, s, sc, cam, out, passes, counters,
+                           next_counters, tiles_x, ntiles, ovf_ref, ovf_t, ovf_stride, stat, next_stat);
     else
-        hipLaunchKernelGGL(aov_kernel<false>, dim3(grid), dim3(256), 0, s, sc, cam, out, passes, counters, tiles_x,
-                           ntiles, ovf_ref, ovf_t, ovf_stride, stat);
+        hipLaunchKernelGGL(aov_kernel<false>, dim3(grid), dim3(256), 0, s, sc, cam, out, passes, counters,
+                           next_counters, tiles_x, ntiles, ovf_ref, ovf_t, ovf_stride, stat, next_stat);
     return hipGetLastError();
 }
 

@@ -103,6 +103,15 @@ void check_device(rt_scene *s) {
     s->cu_count = prop.multiProcessorCount;
 }
 
+void alloc_aov_counters(rt_scene *s) {
+    RT_HIP(hipMalloc(&s->d_aov_ctr, 2 * 256 * sizeof(uint32_t)));
+    RT_HIP(hipMalloc(&s->d_aov_stat, 2 * 8 * sizeof(unsigned long long)));
+    RT_HIP(hipMemset(s->d_aov_ctr, 0, 2 * 256 * sizeof(uint32_t)));
+    RT_HIP(hipMemset(s->d_aov_stat, 0, 2 * 8 * sizeof(unsigned long long)));
+    RT_HIP(hipDeviceSynchronize());   // the scene's stream does not wait on the null stream
+    s->aov_parity = 0;
+}
+
 void ensure_ovf(rt_scene *s, uint32_t threads) {
     uint32_t deep = s->max_stack + 1 > (uint32_t)kLdsStack ? s->max_stack + 1 - kLdsStack : 1;
     size_t need = (size_t)deep * threads * sizeof(uint32_t);
@@ -163,6 +172,8 @@ rt_scene::~rt_scene() {
     for (auto &b : aov_bufs) (void)hipFree(b.p);
     if (d_counters) (void)hipFree(d_counters);
     if (d_stat) (void)hipFree(d_stat);
+    if (d_aov_ctr) (void)hipFree(d_aov_ctr);
+    if (d_aov_stat) (void)hipFree(d_aov_stat);
     if (d_ovf_ref) (void)hipFree(d_ovf_ref);
     if (d_ovf_t) (void)hipFree(d_ovf_t);
     if (d_rays) (void)hipFree(d_rays);
@@ -309,6 +320,7 @@ int rt_scene_create(const rt_scene_desc *desc, rt_scene **out) {
         s->build_ms = bvh.ms;
         RT_HIP(hipMalloc(&s->d_counters, 8 * 32 * sizeof(uint32_t)));
         RT_HIP(hipMalloc(&s->d_stat, 8 * sizeof(unsigned long long)));
+        alloc_aov_counters(s);
         s->stats.build_ms = bvh.ms;
         s->stats.bvh_nodes = s->bvh_nodes;
         s->stats.bvh_depth = s->bvh_depth;
@@ -502,6 +514,7 @@ int rt_scene_load(const char *path, rt_scene **out) {
         s->build_ms = 0.0;
         RT_HIP(hipMalloc(&s->d_counters, 8 * 32 * sizeof(uint32_t)));
         RT_HIP(hipMalloc(&s->d_stat, 8 * sizeof(unsigned long long)));
+        alloc_aov_counters(s);
         s->stats.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         s->stats.bvh_nodes = s->bvh_nodes;
         s->stats.bvh_depth = s->bvh_depth;
@@ -577,12 +590,15 @@ int rt_render_aov(rt_scene *s, const rt_camera *cam, uint32_t passes, const rt_a
         const uint32_t grid = persistent_grid(s, aov_blocks_per_cu(count));
         ensure_ovf(s, grid * 256);
         const int64_t ovf_stride = (int64_t)grid * 256;
-        RT_HIP(launch_reset(s->d_counters, 8 * 32, s->stream, (uint32_t *)s->d_stat, 16));
+        const uint32_t cur = s->aov_parity, nxt = cur ^ 1u;
+        uint32_t *ctr = s->d_aov_ctr + 256 * cur;
+        unsigned long long *stat = s->d_aov_stat + 8 * cur;
         RT_HIP(hipEventRecord(s->ev[0], s->stream));
-        RT_HIP(launch_aov(s->ds, *cam, s->aov, passes, s->d_counters, grid, s->d_ovf_ref, s->d_ovf_t, ovf_stride,
-                          s->d_stat, count, s->stream));
+        RT_HIP(launch_aov(s->ds, *cam, s->aov, passes, ctr, s->d_aov_ctr + 256 * nxt, grid, s->d_ovf_ref, s->d_ovf_t,
+                          ovf_stride, stat, s->d_aov_stat + 8 * nxt, count, s->stream));
         RT_HIP(hipEventRecord(s->ev[1], s->stream));
         RT_HIP(hipStreamSynchronize(s->stream));
+        s->aov_parity = nxt;
         float ms = 0.0f;
         RT_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
         rt_stats &st = s->stats;
@@ -595,7 +611,7 @@ int rt_render_aov(rt_scene *s, const rt_camera *cam, uint32_t passes, const rt_a
         st.cache_queries = 0;
         if (count) {
             unsigned long long c[2];
-            RT_HIP(hipMemcpy(c, s->d_stat, sizeof(c), hipMemcpyDeviceToHost));
+            RT_HIP(hipMemcpy(c, stat, sizeof(c), hipMemcpyDeviceToHost));
             st.node_visits = c[0];
             st.tri_tests = c[1];
         } else {

@@ -786,9 +786,12 @@ hipError_t nrc_launch_gen(const DevScene &sc, const NrcGen &gp, float *out, uint
 }
 
 int nrc_gen_blocks_per_cu() {
-    int nb = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nrc_gen_kernel, 256, 0);
-    return nb > 0 ? nb : 1;
+    static const int blocks = [] {
+        int nb = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, nrc_gen_kernel, 256, 0);
+        return nb > 0 ? nb : 1;
+    }();
+    return blocks;
 }
 
 }  // namespace rtamd

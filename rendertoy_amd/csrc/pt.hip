@@ -412,15 +412,18 @@ __global__ void pt_zero_u32_kernel(uint32_t *p, size_t n) {
 }
 
 int pt_trace_blocks_per_cu() {
-    int nb = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false, false, true>, 256, 0);
-    int nb2 = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true, true, true>, 256, 0);
-    int nb3 = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb3, pt_primary_kernel<true>, 256, 0);
-    nb = nb < nb2 ? nb : nb2;
-    nb = nb < nb3 ? nb : nb3;
-    return nb > 0 ? nb : 1;
+    static const int blocks = [] {
+        int nb = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false, false, true>, 256, 0);
+        int nb2 = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true, true, true>, 256, 0);
+        int nb3 = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb3, pt_primary_kernel<true>, 256, 0);
+        nb = nb < nb2 ? nb : nb2;
+        nb = nb < nb3 ? nb : nb3;
+        return nb > 0 ? nb : 1;
+    }();
+    return blocks;
 }
 
 hipError_t pt_enqueue_frame_start(const PtDev &p, size_t ctr_words, hipStream_t s) {

@@ -38,6 +38,11 @@ struct rt_scene {
     // work queue counters (8 x 128-byte lines) and statistics
     uint32_t *d_counters = nullptr;
     unsigned long long *d_stat = nullptr;
+    // AOV launches: two sets of band counters and statistics; launch k uses set
+    // k % 2 and clears the other for launch k + 1, so no reset launch is needed
+    uint32_t *d_aov_ctr = nullptr;            // 2 x 256
+    unsigned long long *d_aov_stat = nullptr; // 2 x 8
+    uint32_t aov_parity = 0;
     // traversal stack overflow region
     uint32_t *d_ovf_ref = nullptr;
     float *d_ovf_t = nullptr;

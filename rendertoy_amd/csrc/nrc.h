@@ -17,5 +17,17 @@ struct rt_cache {
     uint32_t *count = nullptr;     // training-sample counter
     int step = 0;                  // Adam steps taken
     uint32_t frame = 0;            // training-path stream position
+    // training paths of the next rt_cache_train call, generated on their own
+    // stream while this call trains (the generator does not read the network)
+    hipStream_t gen_stream = nullptr;
+    hipEvent_t gen_done = nullptr;
+    float *pre_samples = nullptr;
+    uint32_t *pre_count = nullptr;
+    uint32_t pre_cap = 0;
+    uint32_t *pre_ovf_ref = nullptr;
+    float *pre_ovf_t = nullptr;
+    size_t pre_ovf_entries = 0;
+    bool pre_pending = false;
+    rtamd::NrcGen pre_gp{};
     ~rt_cache();
 };

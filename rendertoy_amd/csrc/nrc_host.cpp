@@ -92,7 +92,12 @@ void train_steps(rt_cache *c, const float *dev_samples, uint32_t n, uint32_t ste
 }  // namespace
 
 void detach_caches(rt_scene *scene) {
-    for (rt_cache *c : scene->caches) c->scene = nullptr;
+    for (rt_cache *c : scene->caches) {
+        // a prefetch still reading the scene's arrays ends before they go
+        if (c->gen_stream) (void)hipStreamSynchronize(c->gen_stream);
+        c->pre_pending = false;
+        c->scene = nullptr;
+    }
     scene->caches.clear();
 }
 
@@ -101,7 +106,10 @@ void detach_caches(rt_scene *scene) {
 using namespace rtamd;
 
 rt_cache::~rt_cache() {
+    if (gen_stream) (void)hipStreamSynchronize(gen_stream);
     for (auto &b : bufs) (void)hipFree(b.p);
+    if (gen_done) (void)hipEventDestroy(gen_done);
+    if (gen_stream) (void)hipStreamDestroy(gen_stream);
 }
 
 extern "C" {
@@ -337,7 +345,7 @@ int rt_cache_train(rt_cache *c, const rt_camera *cam, uint32_t max_depth, uint32
         gp.divH = 1.0f / float(cam->height);
         gp.npix = cam->width * cam->height;
         gp.npaths = c->prm.train_paths;
-        gp.frame = c->frame++;
+        gp.frame = c->frame;
         gp.max_depth = max_depth;
         gp.first_vertex = c->prm.first_vertex;
         gp.capacity = cap;
@@ -345,14 +353,61 @@ int rt_cache_train(rt_cache *c, const rt_camera *cam, uint32_t max_depth, uint32
         gp.key1 = (uint32_t)(c->prm.seed >> 32) ^ 0x9E3779B9u;
         gp.target_clamp = c->prm.target_clamp > 0.0f ? c->prm.target_clamp : 1e30f;
         const uint32_t grid = persistent_grid(s, nrc_gen_blocks_per_cu());
-        ensure_overflow(s, grid * 256);
-        hipchk(nrc_launch_zero((float *)c->count, 1, st), "zero");
-        hipchk(nrc_launch_gen(s->ds, gp, c->samples, c->count, grid, s->d_ovf_ref, s->d_ovf_t, (int64_t)grid * 256,
-                              st),
-               "nrc gen");
         uint32_t made = 0;
-        hipchk(hipMemcpyAsync(&made, c->count, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-        hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        bool have = false;
+        if (c->pre_pending) {
+            // the paths the previous call generated ahead for this frame: the same
+            // samples as generating them now, if the camera and settings match
+            hipchk(hipEventSynchronize(c->gen_done), "hipEventSynchronize");
+            c->pre_pending = false;
+            if (std::memcmp(&c->pre_gp, &gp, sizeof(gp)) == 0) {
+                hipchk(hipMemcpy(&made, c->pre_count, 4, hipMemcpyDeviceToHost), "hipMemcpy");
+                const uint32_t nc = std::min(made, cap);
+                if (nc)
+                    hipchk(hipMemcpyAsync(c->samples, c->pre_samples, (size_t)nc * kNrcFloats * sizeof(float),
+                                          hipMemcpyDeviceToDevice, st),
+                           "hipMemcpyAsync");
+                have = true;
+            }
+        }
+        if (!have) {
+            ensure_overflow(s, grid * 256);
+            hipchk(nrc_launch_zero((float *)c->count, 1, st), "zero");
+            hipchk(nrc_launch_gen(s->ds, gp, c->samples, c->count, grid, s->d_ovf_ref, s->d_ovf_t,
+                                  (int64_t)grid * 256, st),
+                   "nrc gen");
+            hipchk(hipMemcpyAsync(&made, c->count, 4, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+            hipchk(hipStreamSynchronize(st), "hipStreamSynchronize");
+        }
+        ++c->frame;
+        // generate the next frame's paths on the side stream while this call trains
+        {
+            if (!c->gen_stream) {
+                hipchk(hipStreamCreateWithFlags(&c->gen_stream, hipStreamNonBlocking), "hipStreamCreate");
+                hipchk(hipEventCreateWithFlags(&c->gen_done, hipEventDisableTiming), "hipEventCreate");
+                c->pre_count = (uint32_t *)alloc(c, 256);
+            }
+            if (c->pre_cap < cap) {
+                c->pre_samples = (float *)alloc(c, (size_t)cap * kNrcFloats * sizeof(float));
+                c->pre_cap = cap;
+            }
+            const size_t deep = s->max_stack + 1 > (uint32_t)kLdsStack ? s->max_stack + 1 - kLdsStack : 1;
+            const size_t entries = deep * grid * 256;
+            if (c->pre_ovf_entries < entries) {
+                c->pre_ovf_ref = (uint32_t *)alloc(c, entries * sizeof(uint32_t));
+                c->pre_ovf_t = (float *)alloc(c, entries * sizeof(float));
+                c->pre_ovf_entries = entries;
+            }
+            NrcGen nx = gp;
+            nx.frame = c->frame;
+            hipchk(nrc_launch_zero((float *)c->pre_count, 1, c->gen_stream), "zero");
+            hipchk(nrc_launch_gen(s->ds, nx, c->pre_samples, c->pre_count, grid, c->pre_ovf_ref, c->pre_ovf_t,
+                                  (int64_t)grid * 256, c->gen_stream),
+                   "nrc gen");
+            hipchk(hipEventRecord(c->gen_done, c->gen_stream), "hipEventRecord");
+            c->pre_gp = nx;
+            c->pre_pending = true;
+        }
         const uint32_t n = std::min(made, std::min(batch, cap));
         if (n == 0) {
             if (loss_out) *loss_out = 0.0f;

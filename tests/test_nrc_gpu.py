@@ -241,13 +241,32 @@ def test_cache_rejects_bad_params(cornell):
 
 
 def test_cache_outlives_scene():
-    """Destroying the scene first detaches the cache instead of leaving it dangling."""
+    """Destroying the scene first detaches the cache instead of leaving it dangling,
+    also while the next call's training paths are being generated ahead."""
     from rendertoy_amd import capi
     sc = scene("cornell")
     s = capi.Scene(sc)
     c = make_cache(s)
     c.layout()
+    c.train(steps=1)          # leaves the next frame's paths generating on the side stream
     s.close()
     with pytest.raises(capi.RtError):
         c.eval(np.zeros((4, N.RECORD), np.float32))
+    c.close()
+
+
+def test_cache_paths_generated_ahead(cornell):
+    """rt_cache_train generates the next call's paths while it trains.  They are
+    used only when the next call asks for the same camera and depth: switching
+    either regenerates, and training goes on from either kind of call."""
+    sc, s = cornell
+    c = make_cache(s, train_batch=8192, train_paths=8192)
+    W, H = sc.width, sc.height
+    seq = [dict(), dict(), dict(max_depth=3), dict(W=W // 2, H=H // 2), dict(), dict()]
+    losses = [c.train(steps=2, **kw) for kw in seq]
+    print("losses", losses, flush=True)
+    assert all(np.isfinite(losses)) and all(l > 0 for l in losses)
+    q = np.zeros((64, N.RECORD), np.float32)
+    q[:, 6:9] = [0.0, 0.0, 1.0]
+    assert np.all(np.isfinite(c.eval(q)))
     c.close()

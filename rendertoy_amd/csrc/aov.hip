@@ -19,8 +19,7 @@ RT_TRACE_KERNEL void aov_kernel(DevScene sc, rt_camera cam, AovDev out, uint32_t
                                 uint32_t *next_counters, uint32_t tiles_x, uint32_t ntiles, uint32_t *ovf_ref,
                                 float *ovf_t, int64_t ovf_stride, unsigned long long *stat,
                                 unsigned long long *next_stat) {
-    __shared__ uint32_t s_ref[kLdsStack * 256];
-    __shared__ float s_t[kLdsStack * 256];
+    __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
@@ -28,7 +27,7 @@ RT_TRACE_KERNEL void aov_kernel(DevScene sc, rt_camera cam, AovDev out, uint32_t
         next_counters[tid] = 0u;
         if (tid < 8u) next_stat[tid] = 0ull;
     }
-    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
     const float divW = 1.0f / float(cam.width);
     const float divH = 1.0f / float(cam.height);
     const float div_fmn = 1.0f / (cam.far_clip - cam.near_clip);
@@ -95,11 +94,10 @@ RT_TRACE_KERNEL void ray_kernel(DevScene sc, const float *src, const float *dir,
                                                   const uint32_t *exclude, float4 *out, uint32_t n,
                                                   uint32_t *fetch, uint32_t *ovf_ref, float *ovf_t,
                                                   int64_t ovf_stride) {
-    __shared__ uint32_t s_ref[kLdsStack * 256];
-    __shared__ float s_t[kLdsStack * 256];
+    __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
     uint32_t nn = 0, nt = 0;
     BandSched bs = band_sched((n + 63u) / 64u, fetch, sc.share_rays);
     while (true) {

@@ -142,18 +142,35 @@ constexpr int kLdsStack = 12;
 constexpr int kTraceWavesPerSimd = (160 * 1024) / (kLdsStack * 8 * 256);
 #define RT_TRACE_KERNEL __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd)))
 
+typedef __attribute__((address_space(3))) uint64_t *LdsU64;   // LDS, so its loads never become generic loads
+typedef __attribute__((address_space(1))) uint32_t *GlobalU32;
+typedef __attribute__((address_space(1))) float *GlobalF32;
+
+// A 16-byte scene record read through a global-address-space pointer: scene
+// arrays reached through a table (batched scenes) would otherwise be read
+// with generic loads.
+__device__ __forceinline__ float4 gload4(const void *p, int i) {
+    const __attribute__((address_space(1))) float *q = (const __attribute__((address_space(1))) float *)p + 4 * i;
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ uint4 gload4u(const void *p, int i) {
+    const __attribute__((address_space(1))) uint32_t *q =
+        (const __attribute__((address_space(1))) uint32_t *)p + 4 * i;
+    return make_uint4(q[0], q[1], q[2], q[3]);
+}
+
 struct Stack {
-    uint32_t *lref;
-    float *lt;
+    LdsU64 lds;      // (entry t bits << 32) | node ref: one 64-bit LDS access per push or pop
     int lstride;
-    uint32_t *gref;
-    float *gt;
+    GlobalU32 gref;
+    GlobalF32 gt;
     int64_t gstride;
     int sp;
+    __device__ Stack(uint64_t *l, int ls, uint32_t *gr, float *g, int64_t gs, int s0)
+        : lds((LdsU64)l), lstride(ls), gref((GlobalU32)gr), gt((GlobalF32)g), gstride(gs), sp(s0) {}
     __device__ __forceinline__ void push(uint32_t r, float t) {
         if (sp < kLdsStack) {
-            lref[sp * lstride] = r;
-            lt[sp * lstride] = t;
+            lds[sp * lstride] = (uint64_t)r | ((uint64_t)__float_as_uint(t) << 32);
         } else {
             gref[(int64_t)(sp - kLdsStack) * gstride] = r;
             gt[(int64_t)(sp - kLdsStack) * gstride] = t;
@@ -163,8 +180,9 @@ struct Stack {
     __device__ __forceinline__ void pop(uint32_t &r, float &t) {
         --sp;
         if (sp < kLdsStack) {
-            r = lref[sp * lstride];
-            t = lt[sp * lstride];
+            const uint64_t e = lds[sp * lstride];
+            r = (uint32_t)e;
+            t = __uint_as_float((uint32_t)(e >> 32));
         } else {
             r = gref[(int64_t)(sp - kLdsStack) * gstride];
             t = gt[(int64_t)(sp - kLdsStack) * gstride];
@@ -204,7 +222,7 @@ __device__ __forceinline__ bool leaf_test(const float4 *tris, uint32_t ref, V3 o
     const uint32_t cnt = (ref & 7u) + 1u;
     for (uint32_t k = 0; k < cnt; ++k) {
         const float4 *tp = tris + 3 * (size_t)(start + k);
-        const float4 a = tp[0], b = tp[1], c = tp[2];
+        const float4 a = gload4(tp, 0), b = gload4(tp, 1), c = gload4(tp, 2);
         const uint32_t id = __float_as_uint(c.y);
         if (COUNT) ++n_tris;
         if (id == ex0 || (ANY && id == ex1)) continue;
@@ -274,13 +292,10 @@ __device__ __forceinline__ bool trace_step(const Node4 *nodes, const float4 *tri
     if (!(r.cur & kLeafBit)) {
         const char *np = reinterpret_cast<const char *>(nodes + r.cur);
         const uint32_t onx = r.near_off & 0xFFu, ony = (r.near_off >> 8) & 0xFFu, onz = r.near_off >> 16;
-        const float4 nx = *reinterpret_cast<const float4 *>(np + onx);
-        const float4 fx = *reinterpret_cast<const float4 *>(np + (onx ^ 16u));
-        const float4 ny = *reinterpret_cast<const float4 *>(np + ony);
-        const float4 fy = *reinterpret_cast<const float4 *>(np + (ony ^ 16u));
-        const float4 nz = *reinterpret_cast<const float4 *>(np + onz);
-        const float4 fz = *reinterpret_cast<const float4 *>(np + (onz ^ 16u));
-        const uint4 ch = *reinterpret_cast<const uint4 *>(np + offsetof(Node4, child));
+        const float4 nx = gload4(np + onx, 0), fx = gload4(np + (onx ^ 16u), 0);
+        const float4 ny = gload4(np + ony, 0), fy = gload4(np + (ony ^ 16u), 0);
+        const float4 nz = gload4(np + onz, 0), fz = gload4(np + (onz ^ 16u), 0);
+        const uint4 ch = gload4u(np + offsetof(Node4, child), 0);
         if (COUNT) ++n_nodes;
         const float lim = r.best * kSlack;
         float tn[4];
@@ -294,7 +309,7 @@ __device__ __forceinline__ bool trace_step(const Node4 *nodes, const float4 *tri
                                   fmaxf(fmaf(nza[j], r.iz, -r.oz), 0.0f));
             const float b = fminf(fminf(fmaf(fxa[j], r.ix, -r.ox), fmaf(fya[j], r.iy, -r.oy)),
                                   fmaf(fza[j], r.iz, -r.oz));
-            const bool h = rf[j] != kEmptyRef && a <= b * kSlack && a <= lim;
+            const bool h = rf[j] != kEmptyRef && a <= fminf(b * kSlack, lim);
             tn[j] = h ? a : __builtin_inff();
         }
         RT_CSWAP(0, 1);
@@ -344,9 +359,10 @@ __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t e
     stk.sp = 0;
     while (true) {
         if (!(cur & kLeafBit)) {
-            const float4 *np = reinterpret_cast<const float4 *>(sc.nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const uint4 ch = reinterpret_cast<const uint4 *>(np)[6];
+            const Node4 *np = sc.nodes + cur;
+            const float4 lx = gload4(np, 0), hx = gload4(np, 1), ly = gload4(np, 2), hy = gload4(np, 3);
+            const float4 lz = gload4(np, 4), hz = gload4(np, 5);
+            const uint4 ch = gload4u(np, 6);
             if (COUNT) ++n_nodes;
             const float lim = best * kSlack;
             float tn[4];
@@ -361,7 +377,7 @@ __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t e
                 const float t0z = fmaf(lza[j], iz, -oz), t1z = fmaf(hza[j], iz, -oz);
                 const float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
                 const float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
-                const bool h = rf[j] != kEmptyRef && a <= b * kSlack && a <= lim;
+                const bool h = rf[j] != kEmptyRef && a <= fminf(b * kSlack, lim);
                 tn[j] = h ? a : __builtin_inff();
             }
             RT_CSWAP(0, 1);

@@ -595,11 +595,10 @@ __device__ __forceinline__ uint32_t wave_append_n(uint32_t *counter, uint32_t n)
 // adds after vertex k's emission term, dl_k + w_k (e_{k+1} + R_{k+1}).
 __global__ __launch_bounds__(256) void nrc_gen_kernel(DevScene sc, NrcGen gp, float *out, uint32_t *count,
                                                       uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
-    __shared__ uint32_t s_ref[kLdsStack * 256];
-    __shared__ float s_t[kLdsStack * 256];
+    __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
     uint32_t nn = 0, ntt = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     // whole waves iterate together: the sample append is wave-aggregated

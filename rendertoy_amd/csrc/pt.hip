@@ -84,11 +84,10 @@ __device__ __forceinline__ uint32_t find_range(const uint32_t *off, uint32_t n, 
 template <bool BATCH>
 RT_TRACE_KERNEL void pt_primary_kernel(DevScene sc, PtDev p, uint32_t tiles_x, uint32_t ntiles,
                                                          uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
-    __shared__ uint32_t s_ref[kLdsStack * 256];
-    __shared__ float s_t[kLdsStack * 256];
+    __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
     uint32_t nn = 0, nt = 0;
     BandSched bs = band_sched(ntiles, p.prim_ctr, sc.share_prim);
     while (true) {
@@ -308,11 +307,10 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
 template <bool SHADOW, bool BATCH, bool POOL>
 RT_TRACE_KERNEL void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t depth, uint32_t *ovf_ref,
                                      float *ovf_t, int64_t ovf_stride) {
-    __shared__ uint32_t s_ref[kLdsStack * 256];
-    __shared__ float s_t[kLdsStack * 256];
+    __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk{s_ref + tid, s_t + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0};
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
     const uint32_t count = *ctr_at(p, batch, depth, SHADOW ? 2 : 0);
     const uint32_t *q = SHADOW ? p.sq : ((depth & 1u) ? p.q1 : p.q0);
     uint32_t nn = 0, nt = 0;

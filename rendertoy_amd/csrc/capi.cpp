@@ -596,9 +596,9 @@ int rt_render_aov(rt_scene *s, const rt_camera *cam, uint32_t passes, const rt_a
         RT_HIP(hipEventRecord(s->ev[0], s->stream));
         RT_HIP(launch_aov(s->ds, *cam, s->aov, passes, ctr, s->d_aov_ctr + 256 * nxt, grid, s->d_ovf_ref, s->d_ovf_t,
                           ovf_stride, stat, s->d_aov_stat + 8 * nxt, count, s->stream));
+        s->aov_parity = nxt;   // the launch is queued: it uses set `cur` and clears set `nxt`
         RT_HIP(hipEventRecord(s->ev[1], s->stream));
         RT_HIP(hipStreamSynchronize(s->stream));
-        s->aov_parity = nxt;
         float ms = 0.0f;
         RT_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
         rt_stats &st = s->stats;

@@ -122,6 +122,20 @@ RT_TRACE_KERNEL void pt_primary_kernel(DevScene sc, PtDev p, uint32_t tiles_x, u
 // One path vertex: Radiance (src/renderer.cpp:272-333) up to the next ray.
 // pixel is the scene's own pixel index (camera ray, random stream); gpix its
 // index in the frame buffers.
+// World::GetDefaultEmission(-dir) (src/world.cpp:39-42): what a ray that
+// leaves the scene along d sees.
+__device__ __forceinline__ V3 background(const DevScene &sc, V3 d) {
+    return (-d.z < 0.0f) ? ld3(sc.sky) : ld3(sc.ground);
+}
+
+// The last radiance update of a path whose extension ray left the scene (the
+// miss branch of shade_path, for a queued path).
+__device__ __forceinline__ void shade_miss(const DevScene &sc, const PtDev &p, uint32_t slot) {
+    const float4 rd = p.ray_d[slot], tt = p.thr[slot], ll = p.rad[slot];
+    const V3 L = add(V3{ll.x, ll.y, ll.z}, mulv(V3{tt.x, tt.y, tt.z}, background(sc, V3{rd.x, rd.y, rd.z})));
+    p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+}
+
 template <bool FIRST>
 __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &cam, float divW, float divH,
                                            uint32_t gpix, uint32_t pixel, uint32_t sample, const PtDev &p,
@@ -152,9 +166,7 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
         absn = {ab.x, ab.y, ab.z};
     }
     if (hid == kNone) {
-        // World::GetDefaultEmission(-dir) (src/world.cpp:39-42)
-        V3 bg = (-d.z < 0.0f) ? ld3(sc.sky) : ld3(sc.ground);
-        L = add(L, mulv(T, bg));
+        L = add(L, mulv(T, background(sc, d)));
     } else {
         const uint4 ra = philox(make_uint4(pixel, sample, depth, 0u), p.key0, p.key1);
         const uint32_t minfo = sc.info[hid];
@@ -245,6 +257,11 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
     p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
 }
 
+// Shading of the queue for (batch, depth); FIRST: the primary hits, one path
+// per slot.  Queued paths (FIRST = false) are first sorted by the block into
+// misses, which take their background term at once, and hits, which collect
+// in LDS until a full block of them is shaded together: a wave then runs the
+// BSDF code with every lane busy instead of leaving the misses' lanes idle.
 template <bool FIRST, bool BATCH>
 __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t s0,
                                                        uint32_t npaths, uint32_t depth) {
@@ -259,15 +276,12 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
     uint32_t *q_ctr = p.cache_depth ? ctr_at(p, batch, depth, 3) : nullptr;
     __shared__ AppendLds s_app;
     uint32_t it = 0;
-    // Iterate whole blocks so the block-level appends see every lane.
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride, ++it) {
-        const uint32_t j = base + threadIdx.x;
-        const bool active = j < count;
+    // Shades slot (when active) and appends its follow-up work; every thread
+    // of the block makes the call.
+    auto shade = [&](bool active, uint32_t slot) {
         bool push_next = false, push_shadow = false, push_query = false;
         float qf[13];
-        uint32_t slot = 0;
         if (active) {
-            slot = FIRST ? j : q_in[j];
             if constexpr (BATCH) {
                 uint32_t si;
                 if (FIRST) {
@@ -288,7 +302,7 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
             }
         }
         uint32_t idx[3];
-        block_append3(s_app, it, next_ctr, push_next && next_ctr, sh_ctr, push_shadow, q_ctr, push_query, idx);
+        block_append3(s_app, it++, next_ctr, push_next && next_ctr, sh_ctr, push_shadow, q_ctr, push_query, idx);
         if (push_next && next_ctr) q_out[idx[0]] = slot;
         if (push_shadow) p.sq[idx[1]] = slot;
         if (push_query) {
@@ -298,8 +312,56 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
             dst[kRecBounces + 1] = 0.0f;
             dst[kRecSlot] = __uint_as_float(slot);
         }
+    };
+    if constexpr (FIRST) {
+        // Iterate whole blocks so the block-level appends see every lane.
+        for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride)
+            shade(base + threadIdx.x < count, base + threadIdx.x);
+    } else {
+        // Up to 255 hits wait between rounds, plus one block's worth.
+        __shared__ uint32_t s_hits[512];
+        __shared__ uint32_t s_wcnt[4];
+        const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+        uint32_t held = 0;   // block-uniform: hits waiting in s_hits
+        for (uint32_t base = blockIdx.x * blockDim.x;; base += stride) {
+            const bool more = base < count;   // block-uniform
+            if (more) {
+                const uint32_t j = base + threadIdx.x;
+                uint32_t slot = 0;
+                bool hit = false;
+                if (j < count) {
+                    slot = q_in[j];
+                    const DevScene &S = BATCH ? p.bscn[p.sid[slot]].sc : sc;
+                    if (__float_as_uint(p.hit[slot].w) == kNone) shade_miss(S, p, slot);
+                    else hit = true;
+                }
+                const uint64_t m = __ballot(hit);
+                if (lane == 0u) s_wcnt[w] = (uint32_t)__popcll(m);
+                __syncthreads();
+                uint32_t off = held, total = 0;
+                for (uint32_t i = 0; i < 4u; ++i) {
+                    if (i < w) off += s_wcnt[i];
+                    total += s_wcnt[i];
+                }
+                if (hit) s_hits[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = slot;
+                __syncthreads();
+                held += total;
+            } else if (held == 0u) {
+                break;
+            }
+            if (held >= 256u || !more) {
+                // the newest n waiting hits, one per thread; block_append3's
+                // barriers order these reads before the next round's writes
+                const uint32_t n = held < 256u ? held : 256u;
+                const bool act = threadIdx.x < n;
+                const uint32_t slot = act ? s_hits[held - n + threadIdx.x] : 0u;
+                held -= n;
+                shade(act, slot);
+            }
+        }
     }
 }
+
 
 // Extension (SHADOW = false) or shadow rays of the queue for (batch, depth).
 // POOL: lane refill through trace_pool (divergent secondary rays keep more

@@ -126,7 +126,7 @@ def bench_c3(n_frames=2):
     ss = [capi.Scene(sc) for sc in scs]
     samples = sum(sc.width * sc.height * sc.spp for sc in scs)
     b = capi.Batch(ss)
-    for _ in range(3):
+    for _ in range(capi.NUM_TRACE_POLICIES):   # the launch-policy tuner's trials
         b.render(spp=32, max_depth=6, download=False)
     t0 = time.perf_counter()
     rays = 0
@@ -135,7 +135,7 @@ def bench_c3(n_frames=2):
         rays += b.stats()["rays"]
     dt = (time.perf_counter() - t0) / n_frames
     for s in ss:
-        for _ in range(3):   # the tuner's trials
+        for _ in range(capi.NUM_TRACE_POLICIES):   # the tuner's trials
             s.render_pt(spp=32, max_depth=6, download=False)
     t0 = time.perf_counter()
     for s in ss:
@@ -216,8 +216,12 @@ def _rmse(img, ref, ok):
     return float(np.sqrt(np.mean(d * d)))
 
 
-def _timed(fn, reps=3):
-    fn()
+def _timed(fn, reps=3, warm=3):
+    """Median wall time (ms) of fn over reps calls, after `warm` untimed calls:
+    the launch-policy tuner times its three candidate policies on the first
+    three renders of a new frame shape, so those are kept out of the timing."""
+    for _ in range(warm):
+        fn()
     ts = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -238,6 +242,38 @@ def _plain_rmse_at(ms, plain):
     if ms >= ts[-1]:
         return float(rs[-1] * np.sqrt(ts[-1] / ms))
     return float(np.exp(np.interp(np.log(ms), np.log(ts), np.log(rs))))
+
+
+def _time_to_plain_1024(s, c, ref, ref_ok, depth, r1024, train_s):
+    """Wall time for the cached tracer to reach the RMSE of the plain tracer's
+    1,024-spp frame, against that frame's time.  The cached RMSE is modelled as
+    sqrt(b^2 + s^2 / spp) (a bias floor plus variance), fitted at 256 and 1,024
+    spp; the predicted spp is then rendered and measured, and raised by a
+    quarter until its measured RMSE is at or below the target."""
+    img, plain_ms = _timed(lambda: s.render_pt(spp=1024, max_depth=depth, seed=7), reps=1)
+    ok = ref_ok & np.isfinite(img).all(1)
+    goal = _rmse(img, ref, ok)
+    img = c.render_pt(spp=256, max_depth=depth, seed=9)
+    r256 = _rmse(img, ref, ref_ok & np.isfinite(img).all(1))
+    var = max((r256 ** 2 - r1024 ** 2) / (1.0 / 256 - 1.0 / 1024), 0.0)
+    bias2 = max(r1024 ** 2 - var / 1024, 0.0)
+    out = {"plain_1024spp_rmse": goal, "plain_1024spp_ms": round(plain_ms, 2),
+           "cached_bias_floor_rmse": float(np.sqrt(bias2)), "cached_rmse_256spp": r256,
+           "training_s": round(train_s, 3)}
+    if goal * goal <= bias2:
+        out["reached"] = False
+        return out
+    spp = max(1, int(np.ceil(var / (goal * goal - bias2))))
+    for _ in range(4):
+        img, ms = _timed(lambda: c.render_pt(spp=spp, max_depth=depth, seed=10), reps=1)
+        r = _rmse(img, ref, ref_ok & np.isfinite(img).all(1))
+        if r <= goal:
+            break
+        spp = int(np.ceil(spp * 1.25))
+    out.update({"reached": bool(r <= goal), "cached_spp": spp, "cached_rmse": r, "cached_ms": round(ms, 2),
+                "time_ratio_cached_over_plain": round(ms / plain_ms, 3),
+                "time_ratio_with_training": round((ms + train_s * 1e3) / plain_ms, 3)})
+    return out
 
 
 def cache_study(args):
@@ -274,7 +310,7 @@ def cache_study(args):
     for i in range(args.cache_train):
         losses.append(c.train(max_depth=depth, steps=4))
     train_s = time.perf_counter() - t0
-    _, train_call_ms = _timed(lambda: c.train(max_depth=depth, steps=1))
+    _, train_call_ms = _timed(lambda: c.train(max_depth=depth, steps=1), warm=1)
     cached = []
     for spp in (1, 4, 16, 64):
         img, ms = _timed(lambda: c.render_pt(spp=spp, max_depth=depth, seed=7))
@@ -290,6 +326,7 @@ def cache_study(args):
     img = c.render_pt(spp=1024, max_depth=depth, seed=8)
     ok = ref_ok & np.isfinite(img).all(1)
     bias = _rmse(img, ref, ok)
+    target = _time_to_plain_1024(s, c, ref, ref_ok, depth, bias, train_s)
     # inference on a known query count: rt_cache_eval is the only caller of
     # nrc_infer_kernel<0>, one launch per call (its duration comes from the kernel trace)
     nq = 1 << 21
@@ -329,11 +366,40 @@ def cache_study(args):
                "cache": {"levels": 16, "table": 1 << 19, "mlp": "64-64x5-3 ReLU", "train_batch": 65536,
                          "train_calls": args.cache_train, "steps_per_call": 4}},
            "plain": plain, "cached": cached, "cached_1024spp_rmse": bias,
+           "time_to_plain_1024spp_rmse": target,
            "eval_2m_queries_ms_with_pcie": round(eval_ms, 3),
            "train": {"seconds": round(train_s, 3), "call_ms_65536_paths_1_step": round(train_call_ms, 3),
                      "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None},
            "full_resolution_1920x1080": full}
     print(json.dumps(out), flush=True)
+
+
+def profile_run(kind):
+    """Fixed workloads for rocprofv3 counter passes (one JSON line of what ran).
+    pt-c2: the terrain scene at 1920x1080, 16 spp, depth 8, rendered four
+    times (the launch-policy tuner's three trials and its choice).  cache: the
+    same scene with the radiance cache at vertex 1, 20 training calls of 4
+    Adam steps, then two cached renders at 16 spp."""
+    from rendertoy_amd import capi, scenes
+    sc = scenes.terrain_c2()
+    s = capi.Scene(sc)
+    if kind == "pt-c2":
+        n = capi.NUM_TRACE_POLICIES + 1
+        frames = []
+        for i in range(n):
+            s.render_pt(spp=16, seed=i, download=False)
+            st = s.stats()
+            frames.append({"trace_policy": st["trace_policy"], "ms": round(st["total_ms"], 3)})
+        print(json.dumps({"profile": kind, "renders": n, "spp": 16, "depth": sc.max_depth, "frames": frames}),
+              flush=True)
+        return
+    c = capi.Cache(s, train_batch=65536, train_paths=65536, first_vertex=1, seed=3)
+    for _ in range(20):
+        c.train(max_depth=sc.max_depth, steps=4)
+    for i in range(2):
+        c.render_pt(spp=16, seed=i, download=False)
+    print(json.dumps({"profile": kind, "train_calls": 20, "steps_per_call": 4, "renders": 2, "spp": 16}),
+          flush=True)
 
 
 def main():
@@ -353,7 +419,12 @@ def main():
     ap.add_argument("--cache-vertex", type=int, default=1)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (3 times) to calibrate HBM counters")
+    ap.add_argument("--profile", choices=("pt-c2", "cache"), default=None,
+                    help="only run a fixed workload for rocprofv3 counter passes")
     args = ap.parse_args()
+    if args.profile:
+        profile_run(args.profile)
+        return
     if args.copy_calibration:
         import torch   # device-memory plumbing only: one known-size copy kernel
         n = args.copy_calibration // 4
@@ -436,7 +507,7 @@ def main():
         pt = {}
         sc0 = scenes.cornell_c0()
         s0 = capi.Scene(sc0)
-        for _ in range(3):
+        for _ in range(capi.NUM_TRACE_POLICIES):   # the launch-policy tuner's trials
             s0.render_pt(download=False)
         t0 = time.perf_counter()
         n0 = 20
@@ -447,7 +518,8 @@ def main():
         dt = time.perf_counter() - t0
         pt["c0_cornell"] = {"workload": "36-tri Cornell box 256x256, 16 spp, depth 5",
                             "msamples_per_s": round(sc0.width * sc0.height * sc0.spp * n0 / dt / 1e6, 2),
-                            "mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / n0, 3)}
+                            "mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / n0, 3),
+                            "trace_policy": s0.stats()["trace_policy"]}
         if not args.no_cpu:
             pt["c0_cornell"]["cpu_baseline"] = cpu_baseline_c0(sc0, sc0.spp)
             pt["c0_cornell"]["speedup_vs_cpu"] = round(
@@ -455,7 +527,7 @@ def main():
         del s0
         sc2 = scenes.terrain_c2()
         s2 = capi.Scene(sc2)
-        for _ in range(3):   # the launch-policy tuner times its three candidates on the first three frames
+        for _ in range(capi.NUM_TRACE_POLICIES):   # the tuner times each candidate policy on one frame
             s2.render_pt(download=False)
         t0 = time.perf_counter()
         n2 = 3
@@ -466,7 +538,8 @@ def main():
         dt = time.perf_counter() - t0
         pt["c2_terrain"] = {"workload": "2,424,832-tri Perlin terrain + 256 icospheres, 1920x1080, 64 spp, depth 8",
                             "msamples_per_s": round(sc2.width * sc2.height * sc2.spp * n2 / dt / 1e6, 2),
-                            "mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / n2, 2)}
+                            "mrays_per_s": round(rays / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3 / n2, 2),
+                            "trace_policy": s2.stats()["trace_policy"]}
         del s2
         if not args.no_cpu:
             cb = cpu_baseline_c2(sc2)

@@ -144,6 +144,11 @@ enum rt_pt_flags {
                                     frame shape try each policy once and keep the fastest */
 };
 
+/* Trace-queue policies the launch-policy tuner tries, in order, on the first
+ * renders of a new frame shape: static chunks, dynamic chunks, and dynamic
+ * chunks with lane refill.  The image does not depend on the policy. */
+#define RT_PT_NUM_TRACE_POLICIES 3
+
 /* PathTracingRenderer::Render (src/renderer.cpp:236-270) with Radiance and
  * DirectLight (renderer.cpp:272-384): buffer += L/spp per sample, in sample
  * order per pixel.  The frame stays on the device; when rgb_host is not NULL
@@ -250,7 +255,9 @@ typedef struct rt_stats {
     uint64_t samples;             /* camera samples */
     uint64_t node_visits;         /* BVH4 node fetches (only when counting is on) */
     uint64_t tri_tests;           /* triangle tests (only when counting is on) */
-    uint32_t bvh_nodes, bvh_depth, bvh_max_stack, reserved;
+    uint32_t bvh_nodes, bvh_depth, bvh_max_stack;
+    uint32_t trace_policy;        /* path tracing: trace-queue policy of the last render, 0 ..
+                                     RT_PT_NUM_TRACE_POLICIES - 1 (RT_PT_NO_TUNE: 0) */
     double build_ms;              /* host BVH build time at rt_scene_create */
     uint64_t shadow_rays;         /* shadow rays among `rays` */
     uint64_t primary_rays;        /* primary rays among `rays` */

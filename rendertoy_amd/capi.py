@@ -22,6 +22,7 @@ RT_ERR = {1: "RT_ERR_INVALID", 2: "RT_ERR_HIP", 3: "RT_ERR_NO_DEVICE", 4: "RT_ER
 AOV_ID, AOV_T, AOV_NORMAL, AOV_DEPTH, AOV_ALBEDO, AOV_MASK = 1, 2, 4, 8, 16, 32
 AOV_ALL = 63
 PT_NO_GRAPH, PT_NO_TUNE = 1, 4
+NUM_TRACE_POLICIES = 3   # RT_PT_NUM_TRACE_POLICIES
 NONE = 0xFFFFFFFF
 
 
@@ -62,11 +63,11 @@ class rt_stats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double), ("rays", C.c_uint64),
                 ("samples", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("bvh_nodes", C.c_uint32), ("bvh_depth", C.c_uint32), ("bvh_max_stack", C.c_uint32),
-                ("reserved", C.c_uint32), ("build_ms", C.c_double), ("shadow_rays", C.c_uint64),
+                ("trace_policy", C.c_uint32), ("build_ms", C.c_double), ("shadow_rays", C.c_uint64),
                 ("primary_rays", C.c_uint64), ("cache_queries", C.c_uint64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class rt_denoise_params(C.Structure):

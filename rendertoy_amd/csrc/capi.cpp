@@ -609,6 +609,7 @@ int rt_render_aov(rt_scene *s, const rt_camera *cam, uint32_t passes, const rt_a
         st.shadow_rays = 0;
         st.primary_rays = np;
         st.cache_queries = 0;
+        st.trace_policy = 0;
         if (count) {
             unsigned long long c[2];
             RT_HIP(hipMemcpy(c, stat, sizeof(c), hipMemcpyDeviceToHost));
@@ -675,6 +676,7 @@ int rt_intersect(rt_scene *s, uint32_t n, const float *src, const float *dir, co
         s->stats.shadow_rays = 0;
         s->stats.primary_rays = 0;
         s->stats.cache_queries = 0;
+        s->stats.trace_policy = 0;
         return (int)RT_OK;
     });
 }

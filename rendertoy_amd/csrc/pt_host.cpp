@@ -38,7 +38,8 @@ struct PtState {
     // trace-queue policy once and keep the fastest (the image does not depend on it)
     uint64_t tune_key[4] = {0, 0, 0, 0};
     uint32_t tune_trial = 0, tune_best = 0;
-    float tune_ms[4] = {0, 0, 0, 0};
+    float tune_ms[RT_PT_NUM_TRACE_POLICIES] = {};
+    uint32_t last_policy = 0;       // policy of the last render (rt_stats::trace_policy)
 };
 
 // Trace-queue policies tried by the tuner: chunks all static for small queues
@@ -49,6 +50,7 @@ struct TracePolicy {
 };
 static const TracePolicy kTracePolicies[] = {{kSchedAuto, 64u}, {0u, 64u}, {0u, kRayRefill}};
 constexpr uint32_t kNumTraceShares = sizeof(kTracePolicies) / sizeof(kTracePolicies[0]);
+static_assert(kNumTraceShares == RT_PT_NUM_TRACE_POLICIES, "policy count in the C ABI header");
 
 namespace {
 
@@ -191,6 +193,7 @@ static int render_frame(rt_scene *s, const rt_camera *cam, rt_batch *batch, cons
         PtDev &p = st->dev;
         // trace-queue policy: fixed, or the tuner's trial / choice for this workload shape
         uint32_t trial = kNumTraceShares;
+        uint32_t policy = 0;
         if (prm->flags & RT_PT_NO_TUNE) {
             p.share_trace = kTracePolicies[0].share;
             p.refill = kTracePolicies[0].refill;
@@ -202,10 +205,12 @@ static int render_frame(rt_scene *s, const rt_camera *cam, rt_batch *batch, cons
                 st->tune_trial = 0;
             }
             if (st->tune_trial < kNumTraceShares) trial = st->tune_trial;
-            const TracePolicy &tp = kTracePolicies[trial < kNumTraceShares ? trial : st->tune_best];
+            policy = trial < kNumTraceShares ? trial : st->tune_best;
+            const TracePolicy &tp = kTracePolicies[policy];
             p.share_trace = tp.share;
             p.refill = tp.refill;
         }
+        st->last_policy = policy;
         if (batch) {
             std::memset(&p.cam, 0, sizeof(p.cam));
             p.divW = p.divH = 0.0f;
@@ -330,6 +335,7 @@ static int render_frame(rt_scene *s, const rt_camera *cam, rt_batch *batch, cons
         sts.primary_rays = npix;
         sts.samples = (uint64_t)npix * prm->spp;
         sts.cache_queries = queries;
+        sts.trace_policy = policy;
         sts.node_visits = sts.tri_tests = 0;
         if (!batch && rgb_host)
             hipchk(hipMemcpy(rgb_host, p.fb, (size_t)npix * 12, hipMemcpyDeviceToHost), "hipMemcpy");

@@ -30,8 +30,8 @@ def test_batch_equals_single_renders(members, batch_paths):
     print("case batch_paths", batch_paths, flush=True)
     b = capi.Batch(members)
     refs = [s.render_pt(spp=4, max_depth=5, seed=11) for s in members]
-    # three renders: the tuner's trials, every trace-queue policy once
-    for trial in range(3):
+    # the tuner's trials: every trace-queue policy once
+    for trial in range(capi.NUM_TRACE_POLICIES):
         outs = b.render(spp=4, max_depth=5, seed=11, batch_paths=batch_paths)
         for s, o, ref in zip(members, outs, refs):
             assert np.array_equal(bits(o), bits(ref)), (s.sc.name, trial)

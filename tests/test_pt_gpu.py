@@ -47,20 +47,22 @@ def test_pt_matches_oracle(orc, name, spp, depth):
 def test_pt_deterministic_across_launch_shapes():
     """Same seed: bit-identical frames for different batch sizes, with and
     without the hipGraph, on a second replay of the graph, and under every
-    trace-queue policy: the first three renders of a workload are the tuner's
-    trials (static chunks, dynamic chunks, dynamic with lane refill)."""
+    trace-queue policy: the first renders of a workload are the tuner's trials
+    (static chunks, dynamic chunks, dynamic with lane refill)."""
     from rendertoy_amd import capi
     sc = scene("cornell")
     s = capi.Scene(sc)
-    a = s.render_pt(spp=8, max_depth=5, seed=7)                       # trial: static chunks
-    b = s.render_pt(spp=8, max_depth=5, seed=7)                       # trial: dynamic chunks
-    r = s.render_pt(spp=8, max_depth=5, seed=7)                       # trial: lane refill
+    trials = []
+    for k in range(capi.NUM_TRACE_POLICIES):
+        trials.append(s.render_pt(spp=8, max_depth=5, seed=7))
+        assert s.stats()["trace_policy"] == k
+    a = trials[0]
     g = s.render_pt(spp=8, max_depth=5, seed=7)                       # the tuner's choice
     h = s.render_pt(spp=8, max_depth=5, seed=7)                       # graph replay
     c = s.render_pt(spp=8, max_depth=5, seed=7, flags=capi.PT_NO_GRAPH)
     d = s.render_pt(spp=8, max_depth=5, seed=7, batch_paths=64 * 64 * 3)   # batches of 3 samples
     f = s.render_pt(spp=8, max_depth=5, seed=7, flags=capi.PT_NO_TUNE)
-    for x in (b, r, g, h, c, d, f):
+    for x in trials[1:] + [g, h, c, d, f]:
         assert np.array_equal(a.view(np.uint32), x.view(np.uint32))
     e = s.render_pt(spp=8, max_depth=5, seed=8)
     assert not np.array_equal(a, e)

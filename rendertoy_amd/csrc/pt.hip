@@ -363,16 +363,14 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
 }
 
 
-// Extension (SHADOW = false) or shadow rays of the queue for (batch, depth).
-// POOL: lane refill through trace_pool (divergent secondary rays keep more
-// lanes busy); otherwise each wave traces its chunk of 64 rays to the end.
+// Extension (SHADOW = false) or shadow rays of the queue for (batch, depth),
+// by every wave of the launch.  POOL: lane refill through trace_pool
+// (divergent secondary rays keep more lanes busy); otherwise each wave traces
+// its chunk of 64 rays to the end.
 template <bool SHADOW, bool BATCH, bool POOL>
-RT_TRACE_KERNEL void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t depth, uint32_t *ovf_ref,
-                                     float *ovf_t, int64_t ovf_stride) {
-    __shared__ uint64_t s_stk[kLdsStack * 256];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u;
-    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
+__device__ __forceinline__ void trace_queue(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t depth,
+                                            Stack &stk) {
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t count = *ctr_at(p, batch, depth, SHADOW ? 2 : 0);
     const uint32_t *q = SHADOW ? p.sq : ((depth & 1u) ? p.q1 : p.q0);
     uint32_t nn = 0, nt = 0;
@@ -440,6 +438,23 @@ RT_TRACE_KERNEL void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint3
     }
 }
 
+// The extension rays of depth ext_depth, then the shadow rays of depth
+// sh_depth (kNone: that part is skipped).  Both queues are written by the
+// shade pass before the launch and touch different path fields (hit; rad), so
+// one launch serves both: waves that run out of extension rays go on to
+// shadow rays while the last extension rays finish, instead of a second
+// launch starting after the first one's tail.
+template <bool BATCH, bool POOL>
+RT_TRACE_KERNEL void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint32_t ext_depth, uint32_t sh_depth,
+                                     uint32_t *ovf_ref, float *ovf_t, int64_t ovf_stride) {
+    __shared__ uint64_t s_stk[kLdsStack * 256];
+    const uint32_t tid = threadIdx.x;
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
+    if (ext_depth != kNone) trace_queue<false, BATCH, POOL>(sc, p, batch, ext_depth, stk);
+    if (sh_depth != kNone) trace_queue<true, BATCH, POOL>(sc, p, batch, sh_depth, stk);
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(256) void pt_accum_kernel(PtDev p, uint32_t K) {
     const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
@@ -474,9 +489,9 @@ __global__ void pt_zero_u32_kernel(uint32_t *p, size_t n) {
 int pt_trace_blocks_per_cu() {
     static const int blocks = [] {
         int nb = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false, false, true>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pt_trace_kernel<false, true>, 256, 0);
         int nb2 = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true, true, true>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb2, pt_trace_kernel<true, true>, 256, 0);
         int nb3 = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb3, pt_primary_kernel<true>, 256, 0);
         nb = nb < nb2 ? nb : nb2;
@@ -513,17 +528,26 @@ static void enqueue_bounce_loop(const DevScene &sc, const PtDev &p, uint32_t bat
                                 uint32_t depth_end, uint32_t trace_grid, uint32_t shade_grid, uint32_t *ovf_ref,
                                 float *ovf_t, int64_t ovf_stride, bool shadows, const PtCache &cache, hipStream_t s,
                                 hipError_t &err) {
+    // one trace launch: extension rays of ext (kNone: none), shadow rays of sh (kNone: none)
+    auto trace = [&](uint32_t ext, uint32_t sh) {
+        if (ext == kNone && sh == kNone) return;
+        if (p.refill < 64u)
+            hipLaunchKernelGGL((pt_trace_kernel<BATCH, true>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch, ext,
+                               sh, ovf_ref, ovf_t, ovf_stride);
+        else
+            hipLaunchKernelGGL((pt_trace_kernel<BATCH, false>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch, ext,
+                               sh, ovf_ref, ovf_t, ovf_stride);
+    };
+    // shade d leaves the shadow queue of depth d and the extension queue of
+    // depth d + 1; both are traced by the launch before shade d + 1
+    uint32_t sh = kNone;   // shadow depth waiting for a trace launch
     for (uint32_t d = 0; d < depth_end; ++d) {
         if (d == 0) {
             hipLaunchKernelGGL((pt_shade_kernel<true, BATCH>), dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0,
                                npaths, 0u);
         } else {
-            if (p.refill < 64u)
-                hipLaunchKernelGGL((pt_trace_kernel<false, BATCH, true>), dim3(trace_grid), dim3(256), 0, s, sc, p,
-                                   batch, d, ovf_ref, ovf_t, ovf_stride);
-            else
-                hipLaunchKernelGGL((pt_trace_kernel<false, BATCH, false>), dim3(trace_grid), dim3(256), 0, s, sc, p,
-                                   batch, d, ovf_ref, ovf_t, ovf_stride);
+            trace(d, sh);
+            sh = kNone;
             hipLaunchKernelGGL((pt_shade_kernel<false, BATCH>), dim3(shade_grid), dim3(256), 0, s, sc, p, batch, s0,
                                npaths, d);
         }
@@ -534,13 +558,9 @@ static void enqueue_bounce_loop(const DevScene &sc, const PtDev &p, uint32_t bat
             if (err != hipSuccess) return;
             break;
         }
-        if (shadows && p.refill < 64u)
-            hipLaunchKernelGGL((pt_trace_kernel<true, BATCH, true>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch,
-                               d, ovf_ref, ovf_t, ovf_stride);
-        else if (shadows)
-            hipLaunchKernelGGL((pt_trace_kernel<true, BATCH, false>), dim3(trace_grid), dim3(256), 0, s, sc, p, batch,
-                               d, ovf_ref, ovf_t, ovf_stride);
+        if (shadows) sh = d;
     }
+    trace(kNone, sh);
 }
 
 hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, uint32_t s0, uint32_t K,

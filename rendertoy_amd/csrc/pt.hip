@@ -9,7 +9,8 @@
 //               ray, BSDF sample for the next ray (compacted into queue d+1).
 //   shadow d -- any-hit visibility of the queued shadow rays; visible rays add
 //               their contribution to the path's radiance.
-//   extend d -- closest hit of queue d.
+//   extend d -- closest hit of queue d.  Extension d + 1 and shadow d run in
+//               one trace launch (they touch different path fields).
 //   accum    -- buffer += L/spp in sample order per pixel (renderer.cpp:261).
 //
 // A path slot i of a batch is pixel i % npix, sample s0 + i / npix.  All

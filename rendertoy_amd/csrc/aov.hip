@@ -27,7 +27,7 @@ RT_TRACE_KERNEL void aov_kernel(DevScene sc, rt_camera cam, AovDev out, uint32_t
         next_counters[tid] = 0u;
         if (tid < 8u) next_stat[tid] = 0ull;
     }
-    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride);
     const float divW = 1.0f / float(cam.width);
     const float divH = 1.0f / float(cam.height);
     const float div_fmn = 1.0f / (cam.far_clip - cam.near_clip);
@@ -97,7 +97,7 @@ RT_TRACE_KERNEL void ray_kernel(DevScene sc, const float *src, const float *dir,
     __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride);
     uint32_t nn = 0, nt = 0;
     BandSched bs = band_sched((n + 63u) / 64u, fetch, sc.share_rays);
     while (true) {

@@ -161,16 +161,22 @@ __device__ __forceinline__ uint4 gload4u(const void *p, int i) {
 
 struct Stack {
     LdsU64 lds;      // (entry t bits << 32) | node ref: one 64-bit LDS access per push or pop
+    LdsU64 top;      // lds + min(sp, kLdsStack) * lstride, kept by push and pop (no multiply)
     int lstride;
     GlobalU32 gref;
     GlobalF32 gt;
     int64_t gstride;
     int sp;
-    __device__ Stack(uint64_t *l, int ls, uint32_t *gr, float *g, int64_t gs, int s0)
-        : lds((LdsU64)l), lstride(ls), gref((GlobalU32)gr), gt((GlobalF32)g), gstride(gs), sp(s0) {}
+    __device__ Stack(uint64_t *l, int ls, uint32_t *gr, float *g, int64_t gs)
+        : lds((LdsU64)l), top((LdsU64)l), lstride(ls), gref((GlobalU32)gr), gt((GlobalF32)g), gstride(gs), sp(0) {}
+    __device__ __forceinline__ void reset() {
+        sp = 0;
+        top = lds;
+    }
     __device__ __forceinline__ void push(uint32_t r, float t) {
         if (sp < kLdsStack) {
-            lds[sp * lstride] = (uint64_t)r | ((uint64_t)__float_as_uint(t) << 32);
+            *top = (uint64_t)r | ((uint64_t)__float_as_uint(t) << 32);
+            top += lstride;
         } else {
             gref[(int64_t)(sp - kLdsStack) * gstride] = r;
             gt[(int64_t)(sp - kLdsStack) * gstride] = t;
@@ -180,7 +186,8 @@ struct Stack {
     __device__ __forceinline__ void pop(uint32_t &r, float &t) {
         --sp;
         if (sp < kLdsStack) {
-            const uint64_t e = lds[sp * lstride];
+            top -= lstride;
+            const uint64_t e = *top;
             r = (uint32_t)e;
             t = __uint_as_float((uint32_t)(e >> 32));
         } else {
@@ -279,7 +286,7 @@ __device__ __forceinline__ void ray_init(RayState &r, V3 o, V3 d, uint32_t ex0, 
     r.ex1 = ex1;
     r.d = d;
     r.o = o;
-    stk.sp = 0;
+    stk.reset();
 }
 
 // One step: the node or leaf r.cur, then (after a leaf, or a node none of
@@ -356,7 +363,7 @@ __device__ __forceinline__ bool trace(const DevScene &sc, V3 o, V3 d, uint32_t e
     uint32_t best_id = kNone;
     float bu = 0.0f, bv = 0.0f;
     uint32_t cur = 0;
-    stk.sp = 0;
+    stk.reset();
     while (true) {
         if (!(cur & kLeafBit)) {
             const Node4 *np = sc.nodes + cur;

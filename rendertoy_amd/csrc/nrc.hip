@@ -598,7 +598,7 @@ __global__ __launch_bounds__(256) void nrc_gen_kernel(DevScene sc, NrcGen gp, fl
     __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride);
     uint32_t nn = 0, ntt = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     // whole waves iterate together: the sample append is wave-aggregated

@@ -88,7 +88,7 @@ RT_TRACE_KERNEL void pt_primary_kernel(DevScene sc, PtDev p, uint32_t tiles_x, u
     __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride);
     uint32_t nn = 0, nt = 0;
     BandSched bs = band_sched(ntiles, p.prim_ctr, sc.share_prim);
     while (true) {
@@ -451,7 +451,7 @@ RT_TRACE_KERNEL void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint3
     __shared__ uint64_t s_stk[kLdsStack * 256];
     const uint32_t tid = threadIdx.x;
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
-    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride, 0);
+    Stack stk(s_stk + tid, (int)blockDim.x, ovf_ref + gtid, ovf_t + gtid, ovf_stride);
     if (ext_depth != kNone) trace_queue<false, BATCH, POOL>(sc, p, batch, ext_depth, stk);
     if (sh_depth != kNone) trace_queue<true, BATCH, POOL>(sc, p, batch, sh_depth, stk);
 }

@@ -30,7 +30,7 @@ def close_fraction(g, o, rel=1e-4):
 
 
 @pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo", 8, 5), ("spheres_small", 2, 3),
-                                            ("cornell", 2, 9)])
+                                            ("cornell", 2, 9), ("cornell", 4, 1), ("cornell", 4, 2)])
 def test_pt_matches_oracle(orc, name, spp, depth):
     from rendertoy_amd import capi
     sc = scene(name)

@@ -120,6 +120,7 @@ void ensure_ovf(rt_scene *s, uint32_t threads) {
     if (s->d_ovf_t) RT_HIP(hipFree(s->d_ovf_t));
     s->d_ovf_ref = nullptr;
     s->d_ovf_t = nullptr;
+    s->ovf_bytes = 0;   // a failed allocation below must not leave a stale size behind
     RT_HIP(hipMalloc(&s->d_ovf_ref, need));
     RT_HIP(hipMalloc(&s->d_ovf_t, need));
     s->ovf_bytes = need;

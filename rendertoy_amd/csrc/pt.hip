@@ -19,7 +19,8 @@
 //
 // Batched mode (BATCH = true) runs the paths of many scenes through the same
 // launches: scene i owns pixels [pix_off_i, pix_off_i + npix_i) of the frame
-// buffers and slots [K pix_off_i, K (pix_off_i + npix_i)) of a batch; a path
+// buffers and slots [K pix_off_i, K (pix_off_i + npix_i)) of a batch of K
+// samples per pixel (the frame's last batch may hold fewer); a path
 // finds its scene once per batch (binary search over the offsets) and keeps
 // the scene index in sid[slot].  Pixel indices in the random stream are the
 // scene's own, so each scene's frame equals its single-scene render.
@@ -284,15 +285,17 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
         float qf[13];
         if (active) {
             if constexpr (BATCH) {
+                // samples in this batch: p.K, or fewer in a frame's last batch
+                const uint32_t kb = npaths / p.npix;
                 uint32_t si;
                 if (FIRST) {
-                    si = find_range(p.pix_off, p.nscenes, p.K, slot);
+                    si = find_range(p.pix_off, p.nscenes, kb, slot);
                     p.sid[slot] = si;
                 } else {
                     si = p.sid[slot];
                 }
                 const BatchScene &b = p.bscn[si];
-                const uint32_t l = slot - p.K * p.pix_off[si];
+                const uint32_t l = slot - kb * p.pix_off[si];
                 const uint32_t px = l % b.npix;
                 shade_path<FIRST>(b.sc, b.cam, b.divW, b.divH, p.pix_off[si] + px, px, s0 + l / b.npix, p, slot,
                                   depth, term, last, push_next, push_shadow, push_query, qf);

@@ -41,6 +41,22 @@ def test_batch_equals_single_renders(members, batch_paths):
     b.close()
 
 
+def test_batch_last_batch_with_fewer_samples(members):
+    """spp not a multiple of the samples per batch: 3 spp in batches of 2
+    and 1 (batch_paths = 2 x the frame's pixels).  The last batch maps its
+    slots to scenes with its own sample count; every scene's frame still
+    equals its single-scene render."""
+    from rendertoy_amd import capi
+    total = sum(s.sc.width * s.sc.height for s in members)
+    refs = [s.render_pt(spp=3, max_depth=5, seed=21) for s in members]
+    b = capi.Batch(members)
+    for trial in range(capi.NUM_TRACE_POLICIES):
+        outs = b.render(spp=3, max_depth=5, seed=21, batch_paths=2 * total)
+        for s, o, ref in zip(members, outs, refs):
+            assert np.array_equal(bits(o), bits(ref)), (s.sc.name, trial)
+    b.close()
+
+
 def test_batch_cameras_change_between_frames(members):
     """New cameras and resolutions on a live batch (same graph or a new one)."""
     from rendertoy_amd import capi

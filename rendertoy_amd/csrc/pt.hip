@@ -13,7 +13,9 @@
 //               one trace launch (they touch different path fields).
 //   accum    -- buffer += L/spp in sample order per pixel (renderer.cpp:261).
 //
-// A path slot i of a batch is pixel i % npix, sample s0 + i / npix.  All
+// A path slot i of a batch of K samples per pixel is pixel i / K, sample
+// s0 + i % K: the samples of a pixel share its primary hit, so their first
+// shadow and extension rays leave one point and a wave holds few origins.  All
 // randomness is Philox4x32-10 on (pixel, sample, depth, k), so the image does
 // not depend on batch size, queue order or launch shape.
 //
@@ -296,12 +298,13 @@ __global__ __launch_bounds__(256) void pt_shade_kernel(DevScene sc, PtDev p, uin
                 }
                 const BatchScene &b = p.bscn[si];
                 const uint32_t l = slot - kb * p.pix_off[si];
-                const uint32_t px = l % b.npix;
-                shade_path<FIRST>(b.sc, b.cam, b.divW, b.divH, p.pix_off[si] + px, px, s0 + l / b.npix, p, slot,
+                const uint32_t px = l / kb;
+                shade_path<FIRST>(b.sc, b.cam, b.divW, b.divH, p.pix_off[si] + px, px, s0 + l % kb, p, slot,
                                   depth, term, last, push_next, push_shadow, push_query, qf);
             } else {
-                const uint32_t px = slot % p.npix;
-                shade_path<FIRST>(sc, p.cam, p.divW, p.divH, px, px, s0 + slot / p.npix, p, slot, depth, term, last,
+                const uint32_t kb = npaths / p.npix;
+                const uint32_t px = slot / kb;
+                shade_path<FIRST>(sc, p.cam, p.divW, p.divH, px, px, s0 + slot % kb, p, slot, depth, term, last,
                                   push_next, push_shadow, push_query, qf);
             }
         }
@@ -459,26 +462,39 @@ RT_TRACE_KERNEL void pt_trace_kernel(DevScene sc, PtDev p, uint32_t batch, uint3
     if (sh_depth != kNone) trace_queue<true, BATCH, POOL>(sc, p, batch, sh_depth, stk);
 }
 
-template <bool BATCH>
-__global__ __launch_bounds__(256) void pt_accum_kernel(PtDev p, uint32_t K) {
-    const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
-    if (px >= p.npix) return;
+// buffer += L/spp for the K samples of this batch, in sample order per pixel.
+// A pixel's K records are contiguous (slot K px + k); a block of one wave
+// stages them for its 64 pixels through LDS in rows of up to 16 samples, so
+// the loads are contiguous runs and each lane then sums its own row in order.
+constexpr uint32_t kAccumRow = 17;   // 16 samples, padded against LDS bank conflicts
+__global__ __launch_bounds__(64) void pt_accum_kernel(PtDev p, uint32_t K) {
+    __shared__ float4 tile[64 * kAccumRow];
+    const uint32_t lane = threadIdx.x, px0 = blockIdx.x * 64u;
+    const uint32_t np = p.npix - px0 < 64u ? p.npix - px0 : 64u;   // pixels of this block
+    const uint32_t px = px0 + lane;
     const float inv = (float)p.spp;
-    // slots of pixel px for samples k = 0..K-1: base + k * stride
-    size_t base = px, stride = p.npix;
-    if constexpr (BATCH) {
-        const uint32_t si = find_range(p.pix_off, p.nscenes, 1u, px);
-        stride = p.bscn[si].npix;
-        base = (size_t)K * p.pix_off[si] + (px - p.pix_off[si]);
+    V3 acc = v3(0.0f);
+    if (lane < np) acc = V3{p.fb[3 * (size_t)px], p.fb[3 * (size_t)px + 1], p.fb[3 * (size_t)px + 2]};
+    const size_t base = (size_t)K * px0;
+    for (uint32_t k0 = 0; k0 < K; k0 += 16u) {
+        const uint32_t kc = K - k0 < 16u ? K - k0 : 16u;
+        for (uint32_t e = lane; e < np * kc; e += 64u) {
+            const uint32_t pp = e / kc, kk = e - pp * kc;
+            tile[pp * kAccumRow + kk] = p.rad[base + (size_t)pp * K + k0 + kk];
+        }
+        __syncthreads();
+        if (lane < np)
+            for (uint32_t kk = 0; kk < kc; ++kk) {
+                const float4 l = tile[lane * kAccumRow + kk];
+                acc = add(acc, divs(V3{l.x, l.y, l.z}, inv));
+            }
+        __syncthreads();
     }
-    V3 acc = {p.fb[3 * (size_t)px], p.fb[3 * (size_t)px + 1], p.fb[3 * (size_t)px + 2]};
-    for (uint32_t k = 0; k < K; ++k) {
-        const float4 l = p.rad[base + (size_t)k * stride];
-        acc = add(acc, divs(V3{l.x, l.y, l.z}, inv));
+    if (lane < np) {
+        p.fb[3 * (size_t)px] = acc.x;
+        p.fb[3 * (size_t)px + 1] = acc.y;
+        p.fb[3 * (size_t)px + 2] = acc.z;
     }
-    p.fb[3 * (size_t)px] = acc.x;
-    p.fb[3 * (size_t)px + 1] = acc.y;
-    p.fb[3 * (size_t)px + 2] = acc.z;
 }
 
 __global__ void pt_zero_kernel(float *p, size_t n) {
@@ -582,10 +598,7 @@ hipError_t pt_enqueue_batch(const DevScene &sc, const PtDev &p, uint32_t batch, 
         enqueue_bounce_loop<false>(sc, p, batch, s0, npaths, depth_end, trace_grid, shade_grid, ovf_ref, ovf_t,
                                    ovf_stride, shadows, cache, s, err);
     if (err != hipSuccess) return err;
-    if (p.nscenes)
-        hipLaunchKernelGGL(pt_accum_kernel<true>, dim3((p.npix + 255) / 256), dim3(256), 0, s, p, K);
-    else
-        hipLaunchKernelGGL(pt_accum_kernel<false>, dim3((p.npix + 255) / 256), dim3(256), 0, s, p, K);
+    hipLaunchKernelGGL(pt_accum_kernel, dim3((p.npix + 63) / 64), dim3(64), 0, s, p, K);
     return hipGetLastError();
 }
 

@@ -192,8 +192,7 @@ def make_camera(cam, W, H) -> rt_camera:
     """rt_camera from a scenes.Camera: PrepareScreenSpace (renderer.cpp:103-124)."""
     top, right = cam.screen_space(W, H)
     c = rt_camera()
-    for i in range(16):
-        c.o2w[i] = float(np.float32(cam.o2w[i]))
+    c.o2w[:] = np.asarray(cam.o2w, dtype=np.float32).ravel()[:16].tolist()
     c.top, c.right = float(top), float(right)
     c.near_clip, c.far_clip = float(np.float32(cam.near)), float(np.float32(cam.far))
     c.width, c.height = W, H

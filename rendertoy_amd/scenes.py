@@ -24,6 +24,7 @@ Workloads (sizes fixed by the benchmark definitions):
 from __future__ import annotations
 
 import dataclasses
+import functools
 import hashlib
 import math
 from dataclasses import dataclass, field
@@ -66,17 +67,22 @@ class Camera:
 
     def screen_space(self, W: int, H: int):
         """Renderer::PrepareScreenSpace (src/renderer.cpp:103-124) in float32."""
-        f32 = np.float32
-        gx, gy, fl = f32(self.gate[0]), f32(self.gate[1]), f32(self.focal)
-        top = (gy * f32(25.4) / f32(2.0)) / fl
-        right = (gx * f32(25.4) / f32(2.0)) / fl
-        gate_aspr = gx / gy
-        film_aspr = f32(W) / f32(H)
-        if gate_aspr >= film_aspr:
-            right = right * (film_aspr / gate_aspr)
-        else:
-            top = top * (gate_aspr / film_aspr)
-        return f32(top), f32(right)
+        return _screen_space(float(self.gate[0]), float(self.gate[1]), float(self.focal), int(W), int(H))
+
+
+@functools.lru_cache(maxsize=256)
+def _screen_space(gate_x: float, gate_y: float, focal: float, W: int, H: int):
+    f32 = np.float32
+    gx, gy, fl = f32(gate_x), f32(gate_y), f32(focal)
+    top = (gy * f32(25.4) / f32(2.0)) / fl
+    right = (gx * f32(25.4) / f32(2.0)) / fl
+    gate_aspr = gx / gy
+    film_aspr = f32(W) / f32(H)
+    if gate_aspr >= film_aspr:
+        right = right * (film_aspr / gate_aspr)
+    else:
+        top = top * (gate_aspr / film_aspr)
+    return f32(top), f32(right)
 
 
 @dataclass

@@ -101,9 +101,10 @@ def cpu_baseline_c0(sc, spp):
 
 
 def cpu_baseline_c2(sc, W=24, H=14, spp=16):
-    """RenderToy's PathTracingRenderer on a downscaled C2 frame (same camera and
-    scene; OpenMP runs over samples, so spp = 16 keeps 16 threads busy).  The
-    reference's depth is fixed at 5, the device renders C2 at depth 8."""
+    """RenderToy's PathTracingRenderer on a downscaled frame of the terrain scene
+    (same camera and scene; OpenMP runs over samples, so spp = 16 keeps 16
+    threads busy).  The reference's depth is fixed at 5, the device renders the
+    terrain at depth 8."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import py_oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -488,7 +489,7 @@ def main():
         "metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "C1 primary-ray AOV raycast: 1,048,576 triangles (64 geodesic spheres x 16,384), "
+        "config": {"workload": "primary-ray AOV raycast: 1,048,576 triangles (64 geodesic spheres x 16,384), "
                                "1920x1080 x 1 spp, normal+albedo+depth+id+t passes",
                    "triangles": sc.num_triangles, "width": sc.width, "height": sc.height, "spp": 1,
                    "scene_sha256_16": sc.fingerprint(), "device": arch, "compute_units": cus},
@@ -502,7 +503,7 @@ def main():
                              "from L2/MALL (traffic = measured HBM bytes per launch), so frac can exceed 1; "
                              "the loop is bound by VALU issue and lane divergence (valu, profiles/)"},
     }
-    log("C1: %.1f Msamples/s, kernel %.4f ms, %.1f GB/s algorithmic" % (value, k_ms, achieved))
+    log("AOV raycast: %.1f Msamples/s, kernel %.4f ms, %.1f GB/s algorithmic" % (value, k_ms, achieved))
     if not args.no_pt:
         pt = {}
         sc0 = scenes.cornell_c0()

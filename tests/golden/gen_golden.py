@@ -195,7 +195,7 @@ def part_camera():
 
 
 def part_converged():
-    """C0 Cornell box at 256x256 rendered by K single-threaded processes with
+    """The 36-triangle Cornell box at 256x256 rendered by K single-threaded processes with
     different warm-up offsets of the global engine; stores the mean image and
     the per-pixel standard deviation of the K independent estimates."""
     procs, spp = CONVERGED["procs"], CONVERGED["spp_each"]

@@ -1,6 +1,6 @@
 """Plain path tracer vs RenderToy's converged image (statistical parity).
 
-tests/golden/converged_c0.npz holds the C0 Cornell box (256x256, depth 5)
+tests/golden/converged_c0.npz holds the 36-triangle Cornell box (256x256, depth 5)
 rendered by PathTracingRenderer::Render itself in 8 single-threaded processes
 of 128 spp each (independent mt19937 offsets): the mean image of 1024 spp and
 the per-pixel standard deviation of a 128-spp estimate.  The GPU renders the

@@ -55,7 +55,8 @@ def test_aov_c1_full_frame_sampled(orc):
     s = capi.Scene(sc)
     g = s.render_aov()
     st = s.stats()
-    print("AOV kernel ms (1,048,576 triangles, 1920x1080)", st["kernel_ms"], "bvh", st["bvh_nodes"], st["bvh_depth"], st["bvh_max_stack"], flush=True)
+    print("AOV kernel ms (1,048,576 triangles, 1920x1080)", st["kernel_ms"], "bvh", st["bvh_nodes"],
+          st["bvh_depth"], st["bvh_max_stack"], flush=True)
     rng = np.random.default_rng(5)
     pix = np.sort(rng.choice(sc.width * sc.height, 1024, replace=False)).astype(np.uint32)
     o = orc.scene(sc).render_aov(pix=pix, mode=0, threads=16)

@@ -160,14 +160,18 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
         absn = v3(0.0f);
     } else {
         float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
-        float4 tt = p.thr[slot], ll = p.rad[slot], ab = p.absn[slot];
+        float4 tt = p.thr[slot], ll = p.rad[slot];
         o = {ro.x, ro.y, ro.z};
         last_pdf = ro.w;
         d = {rd.x, rd.y, rd.z};
         ht = h.x; hu = h.y; hv = h.z; hid = __float_as_uint(h.w);
         T = {tt.x, tt.y, tt.z};
         L = {ll.x, ll.y, ll.z};
-        absn = {ab.x, ab.y, ab.z};
+        absn = v3(0.0f);
+        if (tt.w != 0.0f) {   // the path is inside an absorbing medium (see the store below)
+            const float4 ab = p.absn[slot];
+            absn = {ab.x, ab.y, ab.z};
+        }
     }
     if (hid == kNone) {
         L = add(L, mulv(T, background(sc, d)));
@@ -254,8 +258,13 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
             p.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(hid));
         }
         if (push_next) {
-            p.thr[slot] = make_float4(T.x, T.y, T.z, 0.0f);
-            p.absn[slot] = make_float4(absn.x, absn.y, absn.z, 0.0f);
+            // absorption is zero for paths outside media (always, with no transmissive
+            // material): thr.w says whether absn holds anything, which saves its
+            // 16-byte store here and load at the next vertex.  A zero of either sign
+            // gives Beer-Lambert factor 1 (beer).
+            const bool absorbing = absn.x != 0.0f || absn.y != 0.0f || absn.z != 0.0f;
+            p.thr[slot] = make_float4(T.x, T.y, T.z, absorbing ? 1.0f : 0.0f);
+            if (absorbing) p.absn[slot] = make_float4(absn.x, absn.y, absn.z, 0.0f);
         }
     }
     p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);

@@ -44,6 +44,31 @@ def test_pt_matches_oracle(orc, name, spp, depth):
     np.testing.assert_allclose(g.mean(0), o.mean(0), rtol=2e-3, atol=1e-5)
 
 
+def test_pt_absorbing_glass_matches_oracle(orc):
+    """Paths inside an absorbing medium: the material zoo's glass given
+    extinction (0.8, 0.6, 0.9) over at_distance 0.5, so refracted paths carry a
+    nonzero absorption from vertex to vertex (Beer-Lambert, renderer.cpp
+    through RayState::absorption) -- the path state the GPU stores only for
+    paths that are absorbing."""
+    import dataclasses
+    from rendertoy_amd import capi, scenes
+    base = scene("zoo")
+    mats = np.array(base.mats, copy=True)
+    glass = scenes.principled_bsdf((1.0, 1.0, 1.0), roughness=0.05, spec_trans=1.0, ior=1.33, at_distance=0.5,
+                                   extinction=(0.8, 0.6, 0.9))
+    mats[5] = glass
+    sc = dataclasses.replace(base, mats=mats, name="zoo_absorbing")
+    g = capi.Scene(sc).render_pt(spp=8, max_depth=6, seed=99)
+    o, _ = orc.scene(sc).render_pt(spp=8, max_depth=6, rng="philox", seed=99, mode=0, threads=16)
+    plain = capi.Scene(base).render_pt(spp=8, max_depth=6, seed=99)
+    assert np.isfinite(g).all()
+    frac = close_fraction(g, o)
+    print("close fraction", frac, "means", g.mean(0), o.mean(0), "clear glass", plain.mean(0), flush=True)
+    assert frac >= 0.99
+    np.testing.assert_allclose(g.mean(0), o.mean(0), rtol=2e-3, atol=1e-5)
+    assert not np.array_equal(g, plain)   # the absorption changed the image
+
+
 def test_pt_deterministic_across_launch_shapes():
     """Same seed: bit-identical frames for different batch sizes, with and
     without the hipGraph, on a second replay of the graph, and under every

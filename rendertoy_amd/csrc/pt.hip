@@ -160,21 +160,29 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
         absn = v3(0.0f);
     } else {
         float4 ro = p.ray_o[slot], rd = p.ray_d[slot], h = p.hit[slot];
-        float4 tt = p.thr[slot], ll = p.rad[slot];
+        float4 tt = p.thr[slot];
         o = {ro.x, ro.y, ro.z};
         last_pdf = ro.w;
         d = {rd.x, rd.y, rd.z};
         ht = h.x; hu = h.y; hv = h.z; hid = __float_as_uint(h.w);
         T = {tt.x, tt.y, tt.z};
-        L = {ll.x, ll.y, ll.z};
+        L = v3(0.0f);   // rad[slot] is loaded only if this vertex adds to it
         absn = v3(0.0f);
         if (tt.w != 0.0f) {   // the path is inside an absorbing medium (see the store below)
             const float4 ab = p.absn[slot];
             absn = {ab.x, ab.y, ab.z};
         }
     }
+    // radiance changes here only by an emission (or background) term; a vertex
+    // whose term is exactly +0 leaves rad[slot] as it is, unread and unwritten
+    bool rad_dirty = FIRST;
     if (hid == kNone) {
+        if (!FIRST) {
+            const float4 ll = p.rad[slot];
+            L = V3{ll.x, ll.y, ll.z};
+        }
         L = add(L, mulv(T, background(sc, d)));
+        rad_dirty = true;
     } else {
         const uint4 ra = philox(make_uint4(pixel, sample, depth, 0u), p.key0, p.key1);
         const uint32_t minfo = sc.info[hid];
@@ -193,7 +201,15 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
         const float area = sc.geo[hid].w;
         const float pdf_e = emission_pdf(pos, o, neg(d), N, area);
         V3 e_term = depth == 0 ? m.emission : muls(m.emission, PowerHeuristic(last_pdf, pdf_e));
-        L = add(L, mulv(T, e_term));
+        const V3 e_add = mulv(T, e_term);
+        if (FIRST) {
+            L = add(L, e_add);
+        } else if ((__float_as_uint(e_add.x) | __float_as_uint(e_add.y) | __float_as_uint(e_add.z)) != 0u) {
+            // L + (+0) = L exactly (L is never -0); a NaN or -0 term is added as before
+            const float4 ll = p.rad[slot];
+            L = add(V3{ll.x, ll.y, ll.z}, e_add);
+            rad_dirty = true;
+        }
         if (term) {
             // the cache stands for everything after this vertex's emission term
             push_query = true;
@@ -267,7 +283,7 @@ __device__ __forceinline__ void shade_path(const DevScene &sc, const rt_camera &
             if (absorbing) p.absn[slot] = make_float4(absn.x, absn.y, absn.z, 0.0f);
         }
     }
-    p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+    if (rad_dirty) p.rad[slot] = make_float4(L.x, L.y, L.z, 0.0f);
 }
 
 // Shading of the queue for (batch, depth); FIRST: the primary hits, one path
